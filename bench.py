@@ -1,0 +1,176 @@
+"""Benchmark of the Cholesky/ERI hot path on one MI355X.
+
+One step = one full two-step Cholesky decomposition of the Coulomb matrix of
+the (H2O)16 water cluster (10 A box, O 8s5p3d1f / H 4s2p1d, 1200 spherical
+AOs) at tau = 1e-6, from basis upload to Rkpq resident in HBM, followed by
+the AO->MO transform of the vectors (random orthonormal C, nocc = 40,
+nvirt = 300, CA/CB/Rkab in device memory).
+
+Prints one JSON line.  value = ERI shell quartets evaluated per second of
+Cholesky build wall time (whole job, one GPU); cholesky_build_s is the
+build wall time itself.  The roofline object covers the ERI class kernels
+(the dominant kernel family): algorithmic FP64 operations (per-class counts
+emitted by gigpu/eri_gen.py times quartets) over their HIP-event time on the
+library stream.  cpu_baseline times the CPU oracle (the plain-C restatement
+of the reference algorithm, OpenMP) on a bounded sample: the full build of
+the (H2O)2 cluster with the same basis.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "gammcor-integrals_amd"))
+
+METRIC = "Cholesky build s + ERI quartets/s, batched/mixed vs plain GPU at tau=1e-6"
+FP64_PEAK_TFLOPS = 78.6
+
+
+class Hip:
+    """Minimal device-memory plumbing for the device-pointer MO transform."""
+
+    def __init__(self):
+        self.L = C.CDLL("libamdhip64.so")
+        self.L.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+        self.L.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        self.L.hipFree.argtypes = [C.c_void_p]
+        self.L.hipDeviceSynchronize.argtypes = []
+
+    def alloc(self, nbytes):
+        p = C.c_void_p()
+        if self.L.hipMalloc(C.byref(p), nbytes) != 0:
+            raise RuntimeError("hipMalloc failed")
+        return p.value
+
+    def h2d(self, dst, arr):
+        if self.L.hipMemcpy(C.c_void_p(dst), arr.ctypes.data_as(C.c_void_p), arr.nbytes, 1) != 0:
+            raise RuntimeError("hipMemcpy failed")
+
+    def free(self, p):
+        self.L.hipFree(C.c_void_p(p))
+
+    def sync(self):
+        self.L.hipDeviceSynchronize()
+
+
+def cpu_baseline(seed: int):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    from gigpu import systems
+    pyoracle.build()
+    b = systems.water_system(2, 10.0, "large", seed=seed)
+    o = pyoracle.Oracle(b)
+    t0 = time.perf_counter()
+    r = o.chol(tau=1e-6)
+    dt = time.perf_counter() - t0
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": r.nquartets / dt, "unit": "quartets/s", "cores": threads, "kind": "port",
+            "sample": "CPU oracle (C/OpenMP restatement), full two-step Cholesky of (H2O)2 with the same "
+                      "basis (%d AOs, %d vectors, %d quartets) in %.2f s" % (b.nao, r.nvecs, r.nquartets, dt)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--nwater", type=int, default=16)
+    ap.add_argument("--seed", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-mo", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (see DESIGN.md, sharding)",
+              file=sys.stderr)
+        sys.exit(2)
+    if int(os.environ.get("RANK", "0")) != 0:
+        return
+
+    from gigpu import lib, systems
+    basis = systems.water_system(args.nwater, 10.0, "large", seed=args.seed)
+    nocc, nvirt = 40, 300
+    rng = np.random.default_rng(args.seed)
+    Cmo, _ = np.linalg.qr(rng.normal(size=(basis.nao, basis.nao)))
+    Cmo = np.asfortranarray(Cmo)
+    ctx = lib.Context(0)
+    hip = Hip()
+    dC = hip.alloc(Cmo.nbytes)
+    hip.h2d(dC, Cmo)
+    dR = None
+
+    def step():
+        nonlocal dR
+        t0 = time.perf_counter()
+        ctx.set_basis(basis)
+        ctx.chol_build(tau=1e-6, flags=lib.CHOL_TIME_ERI)
+        t_build = time.perf_counter() - t0
+        d = ctx.chol_dims()
+        t_mo = 0.0
+        if not args.no_mo:
+            if dR is None:
+                dR = hip.alloc(8 * d["nvecs"] * nocc * nvirt)
+            t1 = time.perf_counter()
+            ctx.mo_transform_device(dC, basis.nao, 1, nocc, dC, basis.nao, nocc + 1, nocc + nvirt, dR)
+            t_mo = time.perf_counter() - t1
+        return t_build, t_mo, d, ctx.chol_timings()
+
+    for _ in range(args.warmup):
+        step()
+    hip.sync()
+    results = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        results.append(step())
+    hip.sync()
+    wall = time.perf_counter() - t0
+
+    t_build = float(np.mean([r[0] for r in results]))
+    t_mo = float(np.mean([r[1] for r in results]))
+    d = results[-1][2]
+    tm = results[-1][3]
+    quartets = d["nquartets"]
+    eri_s = tm["eri_kernel"]
+    eri_flops = tm["eri_flops"]
+    achieved = eri_flops / eri_s / 1e12 if eri_s > 0 else None
+    out = {
+        "metric": METRIC,
+        "value": quartets / t_build,
+        "unit": "quartets/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1000.0 * wall / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic: seeded (H2O)%d cluster, even-tempered basis, random orthonormal MO coefficients" % args.nwater,
+        "config": {"workload": "plain FP64 two-step Cholesky of (H2O)%d, O 8s5p3d1f / H 4s2p1d, tau=1e-6, "
+                               "+ AO->MO transform nocc=40 nvirt=300" % args.nwater,
+                   "nao": basis.nao, "tau": 1e-6, "form": "plain FP64 GPU (batched/mixed forms not built)"},
+        "cholesky_build_s": t_build,
+        "mo_transform_s": t_mo,
+        "nvecs": d["nvecs"],
+        "eri_quartets": quartets,
+        "phases_s": {k: tm[k] for k in ("vdiag", "step1", "eri_step1", "rtr", "micro", "step2", "vpqrs",
+                                        "potrf_trtri", "vectors", "vabrs", "vectors_gemm", "eri_kernel")},
+        "roofline": {"bound": "mfma", "kernel": "ERI class kernels (FP64 VALU; MI355X FP64 vector peak = matrix peak)",
+                     "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": (achieved / FP64_PEAK_TFLOPS) if achieved else None, "traffic": None,
+                     "flops": eri_flops, "kernel_s": eri_s},
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(args.seed)
+    print(json.dumps(out))
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,314 @@
+// C ABI of libgigpu (declarations and reference citations in include/gigpu.h).
+#include "../../include/gigpu.h"
+#include <cstring>
+#include <memory>
+#include "gi_context.h"
+
+namespace gi {
+void chol_build(Context &C, double tau, double kappa, long long max_block_dim, long long block_dim,
+                long long flags);
+void mo_transform(Context &C, const double *dCA, long long lda, long long a0, long long a1, const double *dCB,
+                  long long ldb, long long b0, long long b1, long long max_buffer_mb, long long flags,
+                  double *dRkab);
+}  // namespace gi
+
+struct gi_context {
+  std::unique_ptr<gi::Context> c;
+};
+
+using gi::Error;
+
+#define GI_TRY(...)                                    \
+  try {                                                \
+    __VA_ARGS__;                                       \
+    return 0;                                          \
+  } catch (const gi::Error &e) {                       \
+    gi::set_error(e.msg);                              \
+    return e.code ? e.code : 1;                        \
+  } catch (const std::exception &e) {                  \
+    gi::set_error(std::string("exception: ") + e.what()); \
+    return 99;                                         \
+  }
+
+static gi::Context &ctx_of(gi_context *c) {
+  if (!c || !c->c) throw Error{7, "null gi_context"};
+  GI_HIP(hipSetDevice(c->c->device));
+  return *c->c;
+}
+
+extern "C" {
+
+const char *gi_last_error(void) { return gi::last_error(); }
+
+int gi_create(gi_context **out, int64_t device) {
+  GI_TRY({
+    if (!out) throw Error{7, "null output pointer"};
+    auto *c = new gi_context;
+    try {
+      c->c = std::make_unique<gi::Context>((int)device);
+    } catch (...) {
+      delete c;
+      throw;
+    }
+    *out = c;
+  })
+}
+
+int gi_destroy(gi_context *c) {
+  GI_TRY({ delete c; })
+}
+
+int gi_set_basis(gi_context *cp, int64_t natoms, const double *coords, int64_t nshells,
+                 const int64_t *shell_centers, const int64_t *shell_params, int64_t nparams,
+                 const int64_t *momentum, const int64_t *nprim, int64_t max_nprim, const double *cntr,
+                 const double *exps, int64_t max_nang, const double *norm, int64_t spher) {
+  GI_TRY({
+    gi::Context &C = ctx_of(cp);
+    if (natoms <= 0 || nshells <= 0 || nparams <= 0 || max_nprim <= 0 || max_nang <= 0)
+      throw Error{8, "invalid basis dimensions"};
+    int lmax = 0;
+    for (int64_t p = 0; p < nparams; ++p) {
+      if (momentum[p] < 0 || momentum[p] > gi::MAXL)
+        throw Error{9, "Basis set includes angular momenta unsupported by the Auto2e subroutine"};
+      if (nprim[p] < 1 || nprim[p] > max_nprim) throw Error{8, "invalid number of primitives"};
+      lmax = std::max<int>(lmax, (int)momentum[p]);
+    }
+    if (lmax > 3)
+      throw Error{9, "no ERI kernels for l > 3 in this build (Auto2e supports l <= 5)"};
+    C.natoms = (int)natoms;
+    C.nshells = (int)nshells;
+    C.spher = spher ? 1 : 0;
+    C.lmax = lmax;
+    C.hshells.assign(nshells, gi::DevShell{});
+    C.shell_l.assign(nshells, 0);
+    C.shell_nang.assign(nshells, 0);
+    C.shell_np.assign(nshells, 0);
+    C.shell_loc.assign(nshells, 0);
+    std::vector<double> hexp, hcn, hnorm;
+    int loc = 0;
+    for (int64_t s = 0; s < nshells; ++s) {
+      int64_t a = shell_centers[s] - 1, p = shell_params[s] - 1;
+      if (a < 0 || a >= natoms || p < 0 || p >= nparams) throw Error{8, "shell index out of range"};
+      int l = (int)momentum[p];
+      int np = (int)nprim[p];
+      int nang = C.spher ? (l >= 2 ? 2 * l + 1 : gi::ncart(l)) : gi::ncart(l);
+      if (nang > max_nang) throw Error{8, "max_nang smaller than a shell's angular function count"};
+      gi::DevShell d{};
+      d.x = coords[3 * a];
+      d.y = coords[3 * a + 1];
+      d.z = coords[3 * a + 2];
+      d.l = l;
+      d.np = np;
+      d.prim0 = (int)hexp.size();
+      d.nang = nang;
+      d.norm0 = (int)hnorm.size();
+      for (int i = 0; i < np; ++i) {
+        hexp.push_back(exps[i + max_nprim * p]);
+        hcn.push_back(cntr[i + max_nprim * p]);
+      }
+      for (int i = 0; i < nang; ++i) hnorm.push_back(norm[i + max_nang * p]);
+      C.hshells[s] = d;
+      C.shell_l[s] = l;
+      C.shell_nang[s] = nang;
+      C.shell_np[s] = np;
+      C.shell_loc[s] = loc;
+      loc += nang;
+    }
+    C.nao = loc;
+    C.dshells.upload(C.hshells, C.stream);
+    C.dexps.upload(hexp, C.stream);
+    C.dcntr.upload(hcn, C.stream);
+    C.dnorms.upload(hnorm, C.stream);
+    GI_HIP(hipStreamSynchronize(C.stream));
+    C.basis_set = true;
+    C.chol.valid = false;
+  })
+}
+
+int gi_eri_quartets(gi_context *cp, int64_t nq, const int64_t *quartets, double kappa, const int64_t *offsets,
+                    double *out, int64_t out_len) {
+  GI_TRY({
+    gi::Context &C = ctx_of(cp);
+    if (!C.basis_set) throw Error{4, "basis not set"};
+    for (int64_t q = 0; q < nq; ++q) {
+      int64_t n = 1;
+      for (int k = 0; k < 4; ++k) {
+        int64_t s = quartets[4 * q + k];
+        if (s < 0 || s >= C.nshells) throw Error{8, "shell index out of range"};
+        n *= C.shell_nang[s];
+      }
+      if (offsets[q] < 0 || offsets[q] + n > out_len) throw Error{8, "output block out of range"};
+    }
+    gi::DBuf<double> dout;
+    dout.alloc(out_len);
+    GI_HIP(hipMemsetAsync(dout.p, 0, sizeof(double) * out_len, C.stream));
+    // group by the bra pair's angular-function count: one leading dimension per group
+    std::map<int, std::vector<int64_t>> groups;
+    for (int64_t q = 0; q < nq; ++q) {
+      int s0 = (int)quartets[4 * q], s1 = (int)quartets[4 * q + 1];
+      groups[C.shell_nang[s0] * C.shell_nang[s1]].push_back(q);
+    }
+    for (auto &g : groups) {
+      std::vector<gi::PairDesc> rd, cd;
+      std::vector<int> rows;
+      for (int64_t q : g.second) {
+        rd.push_back(gi::PairDesc{(int)quartets[4 * q], (int)quartets[4 * q + 1], 0, 0, 0, 0, offsets[q]});
+        cd.push_back(gi::PairDesc{(int)quartets[4 * q + 2], (int)quartets[4 * q + 3], 0, 0, 0, 0, 0});
+        rows.push_back((int)rows.size());
+      }
+      gi::DBuf<gi::PairDesc> drd, dcd;
+      drd.upload(rd, C.stream);
+      dcd.upload(cd, C.stream);
+      gi::EriRequest R;
+      R.rdesc = &rd; R.cdesc = &cd; R.d_rdesc = drd.p; R.d_cdesc = dcd.p;
+      R.rows = &rows; R.cols = &rows; R.mode = gi::LAUNCH_ZIP;
+      R.out = dout.p; R.ldo = g.first; R.kappa = kappa;
+      gi::eri_run(C, R);
+    }
+    GI_HIP(hipMemcpy(out, dout.p, sizeof(double) * out_len, hipMemcpyDeviceToHost));
+  })
+}
+
+int gi_chol_accuracy_tau(int64_t accuracy, double *tau) {
+  GI_TRY({
+    switch (accuracy) {
+      case GI_CHOL_ACCURACY_DEFAULT: *tau = 1.0e-5; break;
+      case GI_CHOL_ACCURACY_TIGHT: *tau = 1.0e-6; break;
+      case GI_CHOL_ACCURACY_LUDICROUS: *tau = 1.0e-7; break;
+      case GI_CHOL_ACCURACY_DEBUG: *tau = 1.0e-10; break;
+      default: throw Error{6, "Invalid accuracy setting"};
+    }
+  })
+}
+
+int gi_chol_build(gi_context *cp, double tau, double kappa, int64_t max_block_dim, int64_t vec_block,
+                  int64_t flags) {
+  GI_TRY({
+    gi::Context &C = ctx_of(cp);
+    if (!(tau > 0.0)) throw Error{6, "tau must be positive"};
+    gi::chol_build(C, tau, kappa, max_block_dim, vec_block, flags);
+  })
+}
+
+int gi_chol_dims(gi_context *cp, int64_t dims[8]) {
+  GI_TRY({
+    gi::Context &C = ctx_of(cp);
+    const gi::CholState &S = C.chol;
+    dims[0] = S.nvecs;
+    dims[1] = S.norb_pairs;
+    dims[2] = S.nshell_pairs;
+    dims[3] = S.nsubsets;
+    dims[4] = S.max_subset_dim;
+    dims[5] = S.npivot_shell_pairs;
+    dims[6] = S.nmacro;
+    dims[7] = S.nquartets;
+  })
+}
+
+static void copy_vec(int64_t *dst, const std::vector<long long> &v) {
+  if (dst && !v.empty()) std::memcpy(dst, v.data(), sizeof(int64_t) * v.size());
+}
+
+int gi_chol_meta(gi_context *cp, int64_t *shell_pairs, int64_t *shell_pair_loc, int64_t *shell_pair_dim,
+                 int64_t *subset_dim, int64_t *subset_bounds, int64_t *pivots, int64_t *pivot_order,
+                 int64_t *pivot_shell_pairs, int64_t *pivot_shell_pair_loc, int64_t *pivot_shell_pair_dim,
+                 int64_t *pivot_orb_pairs) {
+  GI_TRY({
+    gi::Context &C = ctx_of(cp);
+    const gi::CholState &S = C.chol;
+    copy_vec(shell_pairs, S.shell_pairs);
+    copy_vec(shell_pair_loc, S.shell_pair_loc);
+    copy_vec(shell_pair_dim, S.shell_pair_dim);
+    copy_vec(subset_dim, S.subset_dim);
+    copy_vec(subset_bounds, S.subset_bounds);
+    copy_vec(pivots, S.pivots);
+    copy_vec(pivot_order, S.pivot_order);
+    copy_vec(pivot_shell_pairs, S.pivot_shell_pairs);
+    copy_vec(pivot_shell_pair_loc, S.pivot_shell_pair_loc);
+    copy_vec(pivot_shell_pair_dim, S.pivot_shell_pair_dim);
+    copy_vec(pivot_orb_pairs, S.pivot_orb_pairs);
+  })
+}
+
+int gi_chol_rkpq(gi_context *cp, double *dst) {
+  GI_TRY({
+    gi::Context &C = ctx_of(cp);
+    const gi::CholState &S = C.chol;
+    if (!S.valid || !S.rkpq.p) throw Error{50, "Cholesky vectors are not available"};
+    size_t n = (size_t)S.nvecs * S.max_subset_dim * S.nsubsets;
+    GI_HIP(hipMemcpy(dst, S.rkpq.p, sizeof(double) * n, hipMemcpyDeviceToHost));
+  })
+}
+
+int gi_chol_rkpq_device(gi_context *cp, double **ptr) {
+  GI_TRY({
+    gi::Context &C = ctx_of(cp);
+    if (!C.chol.valid || !C.chol.rkpq.p) throw Error{50, "Cholesky vectors are not available"};
+    *ptr = C.chol.rkpq.p;
+  })
+}
+
+int gi_chol_inv_l(gi_context *cp, double *dst) {
+  GI_TRY({
+    gi::Context &C = ctx_of(cp);
+    if (!C.chol.inv_l.p) throw Error{50, "Inv_L is not available"};
+    GI_HIP(hipMemcpy(dst, C.chol.inv_l.p, sizeof(double) * C.chol.nvecs * C.chol.nvecs, hipMemcpyDeviceToHost));
+  })
+}
+
+int gi_chol_vdiag(gi_context *cp, double *dst) {
+  GI_TRY({
+    gi::Context &C = ctx_of(cp);
+    if (C.chol.vdiag.empty()) throw Error{50, "Vdiag is not available"};
+    std::memcpy(dst, C.chol.vdiag.data(), sizeof(double) * C.chol.vdiag.size());
+  })
+}
+
+int gi_chol_timings(gi_context *cp, double t[14]) {
+  GI_TRY({
+    gi::Context &C = ctx_of(cp);
+    auto &T = C.chol.timings;
+    auto g = [&](const char *k) { auto it = T.find(k); return it == T.end() ? 0.0 : it->second; };
+    t[0] = g("step1");
+    t[1] = g("step2");
+    t[2] = g("vectors");
+    t[3] = g("total");
+    t[4] = g("eri_kernel");
+    t[5] = (double)C.chol.eri_flops;
+    t[6] = g("vdiag");
+    t[7] = g("eri_step1");
+    t[8] = g("rtr");
+    t[9] = g("micro");
+    t[10] = g("vpqrs");
+    t[11] = g("potrf_trtri");
+    t[12] = g("vabrs");
+    t[13] = g("vectors_gemm");
+  })
+}
+
+int gi_mo_transform_device(gi_context *cp, const double *CA, int64_t lda, int64_t a0, int64_t a1,
+                           const double *CB, int64_t ldb, int64_t b0, int64_t b1, int64_t max_buffer_mb,
+                           int64_t flags, double *Rkab) {
+  GI_TRY({
+    gi::Context &C = ctx_of(cp);
+    gi::mo_transform(C, CA, lda, a0, a1, CB, ldb, b0, b1, max_buffer_mb, flags, Rkab);
+  })
+}
+
+int gi_mo_transform(gi_context *cp, const double *CA, int64_t lda, int64_t a0, int64_t a1, const double *CB,
+                    int64_t ldb, int64_t b0, int64_t b1, int64_t max_buffer_mb, int64_t flags, double *Rkab) {
+  GI_TRY({
+    gi::Context &C = ctx_of(cp);
+    if (!C.chol.valid) throw Error{50, "Cholesky vectors are not available (run gi_chol_build first)"};
+    int64_t ncolA = a1, ncolB = b1;
+    gi::DBuf<double> dA, dB, dR;
+    dA.upload(CA, (size_t)lda * ncolA, C.stream);
+    dB.upload(CB, (size_t)ldb * ncolB, C.stream);
+    size_t nr = (size_t)C.chol.nvecs * (a1 - a0 + 1) * (b1 - b0 + 1);
+    dR.alloc(nr);
+    gi::mo_transform(C, dA.p, lda, a0, a1, dB.p, ldb, b0, b1, max_buffer_mb, flags, dR.p);
+    GI_HIP(hipMemcpy(Rkab, dR.p, sizeof(double) * nr, hipMemcpyDeviceToHost));
+  })
+}
+
+}  // extern "C"

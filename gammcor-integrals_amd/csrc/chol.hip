@@ -1,0 +1,745 @@
+// Two-step Cholesky decomposition of the Coulomb matrix on one MI355X
+// (Folkestad, Kjonstad, Koch, JCP 150, 194112 (2019)), following the
+// reference driver step by step:
+//   chol_gammcor_CoulombMatrix       Cholesky_Gammcor.f90:380-441
+//   chol2_Step1_FullInterface        TwoStepCholesky.f90:74-167
+//   chol2_Vdiag/DefineBase/Subsets   TwoStepCholesky_Step1.f90:202-530
+//   chol2_Pivots (macro/micro loop)  TwoStepCholesky_Step1.f90:884-1103
+//   chol2_Step2                      TwoStepCholesky.f90:243-364
+//   chol2_PivotShellPairs            TwoStepCholesky_Step2.f90:422-534
+//   chol2_FullDimVectors_Batch       TwoStepCholesky.f90:378-416
+//
+// Everything O(N^3..N^4) runs on the device: ERIs (class kernels), the
+// candidate compaction, M - L^T L (rocBLAS DGEMM), the micro-iteration
+// column update + diagonal update + batch argmax (own kernels), POTRF/TRTRI
+// (rocSOLVER) and Inv_L V^T (strided-batched DGEMM).  The host keeps the
+// bookkeeping whose order fixes the pivot sequence: the shell-pair sorts
+// (Singleton quicksort, identical permutation for tied keys), the BASE
+// screening, the batch selection and the strict-> first-maximum rule.
+#include <rocsolver/rocsolver.h>
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include "gi_context.h"
+
+namespace gi {
+
+void dsort0(double *dx, long long *dy, long long n, int kflag);   // sort.cpp
+
+static double now() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+#define ROCBLAS_CHECK(x)                                                         \
+  do {                                                                          \
+    rocblas_status s_ = (x);                                                    \
+    if (s_ != rocblas_status_success)                                           \
+      throw Error{2, std::string("rocBLAS/rocSOLVER error ") + rocblas_status_to_string(s_) + \
+                         " at " + __FILE__ + ":" + std::to_string(__LINE__)};   \
+  } while (0)
+
+// ------------------------------------------------------------------ kernels
+// out[c] = in[src[c]] for a column vector; for a matrix with nrows rows:
+// out[r + ldo c] = in[r + ldi src[c]] (gather of whole columns)
+__global__ void k_gather_rows_of_cols(double *__restrict__ out, long long ldo, const double *__restrict__ in,
+                                      long long ldi, const long long *__restrict__ src, long long ncols,
+                                      long long nrows) {
+  // out is (ncols_new x nvec): out[c + ldo*r] = in[src[c] + ldi*r]
+  long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  long long r = blockIdx.y;
+  for (; r < nrows; r += gridDim.y)
+    if (c < ncols) out[c + ldo * r] = in[src[c] + ldi * r];
+}
+
+__global__ void k_segmax(const double *__restrict__ D, const long long *__restrict__ loc,
+                         const int *__restrict__ dim, long long n, double *__restrict__ out) {
+  long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const double *p = D + loc[k];
+  double m = p[0];
+  for (int i = 1; i < dim[k]; ++i) m = p[i] > m ? p[i] : m;
+  out[k] = m;
+}
+
+// Near-tie canonicalization of comparison keys (same rule as the oracle):
+// values equal in exact arithmetic differ in their last bits with a sign set
+// by each implementation's rounding; comparing quantized keys makes the
+// pivot order a function of the mathematics.  BASE sort keys: mantissa
+// rounded to 40 bits.  Residual diagonals: absolute quantum
+// delta = 2^(floor(log2 MaxDrs) - 40), a power of two so x/delta is exact.
+static double qrel(double x) {
+  if (x == 0.0 || !std::isfinite(x)) return x;
+  int e;
+  double m = std::frexp(x, &e);
+  return std::ldexp(std::rint(std::ldexp(m, 41)), e - 41);
+}
+__host__ __device__ static inline double qabs(double x, double inv_delta) { return rint(x * inv_delta); }
+
+// first maximum (strict >, scan order) over D[bd[e]], e = 0..nb-1, compared on
+// quantized keys unless inv_delta == 0.  res[0] = raw value, res[1] = (double)e.
+// One workgroup of 1024 threads.
+__global__ void __launch_bounds__(1024) k_batchmax(const double *__restrict__ D, const long long *__restrict__ bd,
+                                                    long long nb, double inv_delta, double *__restrict__ res) {
+  __shared__ double sv[1024];
+  __shared__ long long si[1024];
+  double best = inv_delta > 0.0 ? qabs(-1.0, inv_delta) : -1.0;
+  long long bi = -1;
+  for (long long e = threadIdx.x; e < nb; e += blockDim.x) {
+    double v = D[bd[e]];
+    double k = inv_delta > 0.0 ? qabs(v, inv_delta) : v;
+    if (k > best) { best = k; bi = e; }   // e increases within a thread
+  }
+  sv[threadIdx.x] = best;
+  si[threadIdx.x] = bi;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      double v2 = sv[threadIdx.x + s];
+      long long i2 = si[threadIdx.x + s];
+      double v1 = sv[threadIdx.x];
+      long long i1 = si[threadIdx.x];
+      // keep the larger; on ties the smaller index (first in scan order)
+      bool take = (i2 >= 0) && (i1 < 0 || v2 > v1 || (v2 == v1 && i2 < i1));
+      if (take) { sv[threadIdx.x] = v2; si[threadIdx.x] = i2; }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const long long e = si[0];
+    res[0] = e >= 0 ? D[bd[e]] : -1.0;
+    res[1] = (double)e;
+  }
+}
+
+// one micro iteration (chol2_Lj + D update, TwoStepCholesky_Step1.f90:859-881,1044-1046):
+//   L(:, j) = (M(:, col) - L(:, j0:j-1) L(row, j0:j-1)^T) / sqrt(dmax)
+//   D -= L(:, j)^2
+// L is (ncand x cap) column-major, vectors are columns; col/row/dmax come
+// from the batch argmax result on the device.
+__global__ void __launch_bounds__(256) k_lj(const double *__restrict__ M, long long ldM,
+                                            double *__restrict__ L, long long ncand, long long j0, long long j,
+                                            const long long *__restrict__ bd, const double *__restrict__ bm,
+                                            double *__restrict__ D) {
+  long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= ncand) return;
+  const long long e = (long long)bm[1];
+  const long long row = bd[e];
+  const double sq = sqrt(bm[0]);
+  double s = M[i + ldM * e];
+  const double *Li = L + i;
+  const double *Lr = L + row;
+  for (long long q = j0; q < j; ++q) s -= Li[ncand * q] * Lr[ncand * q];
+  const double l = s / sq;
+  L[i + ncand * j] = l;
+  D[i] = D[i] - l * l;
+}
+
+__global__ void k_blockmax(const double *__restrict__ D, long long n, double *__restrict__ part) {
+  __shared__ double sv[256];
+  double m = -INFINITY;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    m = D[i] > m ? D[i] : m;
+  sv[threadIdx.x] = m;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) sv[threadIdx.x] = fmax(sv[threadIdx.x], sv[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sv[0];
+}
+
+__global__ void k_zero_upper(double *A, long long n) {
+  long long c = blockIdx.y;
+  for (long long r = threadIdx.x + blockIdx.x * (long long)blockDim.x; r < c && r < n; r += (long long)gridDim.x * blockDim.x)
+    A[r + n * c] = 0.0;
+}
+
+// ------------------------------------------------------------------ helpers
+static int pair_dim_of(const Context &C, int sa, int sb) {
+  int na = C.shell_nang[sa], nb = C.shell_nang[sb];
+  return sa != sb ? na * nb : na * (na + 1) / 2;
+}
+
+static double device_max(Context &C, const double *d, long long n, DBuf<double> &tmp) {
+  if (n <= 0) return -INFINITY;
+  int nb = (int)std::min<long long>(1024, (n + 255) / 256);
+  tmp.alloc(1024);
+  hipLaunchKernelGGL(k_blockmax, dim3(nb), dim3(256), 0, C.stream, d, n, tmp.p);
+  std::vector<double> h(nb);
+  GI_HIP(hipMemcpyAsync(h.data(), tmp.p, sizeof(double) * nb, hipMemcpyDeviceToHost, C.stream));
+  GI_HIP(hipStreamSynchronize(C.stream));
+  double m = -INFINITY;
+  for (double v : h) m = v > m ? v : m;
+  return m;
+}
+
+struct EriPhaseTimer {
+  Context &C;
+  std::string name;
+  double ms0;
+  long long q0, f0;
+  EriPhaseTimer(Context &c, const char *n) : C(c), name(n), ms0(c.eri_ms_acc), q0(c.eri_quartets_acc), f0(c.eri_flops_acc) {}
+  ~EriPhaseTimer() {
+    C.chol.eri_kernel_ms[name] += C.eri_ms_acc - ms0;
+    C.chol.eri_quartets[name] += C.eri_quartets_acc - q0;
+  }
+};
+
+// ------------------------------------------------------------------ driver
+void chol_build(Context &C, double tau, double kappa, long long max_block_dim, long long block_dim,
+                long long flags) {
+  if (!C.basis_set) throw Error{4, "basis not set"};
+  if (max_block_dim <= 0 || block_dim <= 0) throw Error{5, "MaxBlockDim and CholVecsBlock must be positive"};
+  CholState &S = C.chol;
+  S.valid = false;
+  S.inv_l.release();
+  S.rkpq.release();
+  S.mo_map.release();
+  S.timings.clear();
+  S.eri_kernel_ms.clear();
+  S.eri_quartets.clear();
+  S.tau = tau;
+  S.kappa = kappa;
+  C.time_eri = (flags & 4) != 0;
+  C.eri_ms_acc = 0;
+  C.eri_quartets_acc = 0;
+  C.eri_flops_acc = 0;
+  const int nsh = C.nshells;
+  const long long nao = C.nao;
+  const long long max_sp = (long long)nsh * (nsh + 1) / 2;
+  const long long max_op = nao * (nao + 1) / 2;
+  const int maxnang = C.spher ? 2 * C.lmax + 1 : ncart(C.lmax);
+  GI_HIP(hipSetDevice(C.device));
+  const double t_begin = now();
+
+  // ---------------- chol2_Vdiag
+  std::vector<long long> vd_loc(max_sp + 1), vd_dim(max_sp + 1);
+  std::vector<int> vd_a(max_sp + 1), vd_b(max_sp + 1);
+  {
+    long long loc = 1, ab = 1;
+    for (int sb = 0; sb < nsh; ++sb)
+      for (int sa = sb; sa < nsh; ++sa) {
+        vd_loc[ab] = loc;
+        vd_dim[ab] = pair_dim_of(C, sa, sb);
+        vd_a[ab] = sa;
+        vd_b[ab] = sb;
+        loc += vd_dim[ab];
+        ++ab;
+      }
+  }
+  DBuf<double> d_vdiag;
+  d_vdiag.alloc(max_op);
+  {
+    double t0 = now();
+    EriPhaseTimer pt(C, "vdiag");
+    std::vector<PairDesc> rd(max_sp), cd(max_sp);
+    std::vector<int> rows(max_sp);
+    for (long long k = 1; k <= max_sp; ++k) {
+      int sa = vd_a[k], sb = vd_b[k];
+      int fl = (sa == sb) ? PAIR_SAME : 0;
+      rd[k - 1] = PairDesc{sa, sb, fl, 0, 0, 0, vd_loc[k] - 1};
+      cd[k - 1] = PairDesc{sa, sb, fl, 0, 0, 0, 0};
+      rows[k - 1] = (int)(k - 1);
+    }
+    DBuf<PairDesc> drd, dcd;
+    drd.upload(rd, C.stream);
+    dcd.upload(cd, C.stream);
+    EriRequest R;
+    R.rdesc = &rd; R.cdesc = &cd; R.d_rdesc = drd.p; R.d_cdesc = dcd.p;
+    R.rows = &rows; R.cols = &rows; R.mode = LAUNCH_DIAG;
+    R.out = d_vdiag.p; R.ldo = 0; R.kappa = kappa;
+    eri_run(C, R);
+    S.vdiag.resize(max_op);
+    GI_HIP(hipMemcpy(S.vdiag.data(), d_vdiag.p, sizeof(double) * max_op, hipMemcpyDeviceToHost));
+    S.timings["vdiag"] = now() - t0;
+  }
+  const std::vector<double> &vdiag = S.vdiag;
+
+  // ---------------- chol2_DefineBase
+  std::vector<double> vd_max(max_sp + 1);
+  std::vector<long long> idxmap(max_sp + 1);
+  for (long long k = 1; k <= max_sp; ++k) {
+    double mx = -INFINITY;
+    for (long long i = 0; i < vd_dim[k]; ++i) mx = std::max(mx, vdiag[vd_loc[k] - 1 + i]);
+    vd_max[k] = mx;
+    idxmap[k] = k;
+  }
+  double maxdrs = -INFINITY;
+  for (long long k = 1; k <= max_sp; ++k) maxdrs = std::max(maxdrs, vd_max[k]);
+  if (!(maxdrs > 0.0)) throw Error{10, "Maximum diagonal Coulomb integral is nonpositive"};
+  double pthresh = std::min(tau, 1.0e-8);
+  pthresh = pthresh * pthresh / maxdrs;
+  const bool raw_ties = (flags & 8) != 0;
+  int maxdrs_exp = 0;
+  std::frexp(maxdrs, &maxdrs_exp);
+  const double inv_delta = std::ldexp(1.0, 41 - maxdrs_exp);   // 1 / 2^(floor(log2 MaxDrs) - 40)
+  if (!raw_ties)
+    for (long long k = 1; k <= max_sp; ++k) vd_max[k] = qrel(vd_max[k]);
+  dsort0(vd_max.data() + 1, idxmap.data() + 1, max_sp, 2);
+  long long k0 = 1;
+  for (long long k = 1; k <= max_sp; ++k) {
+    long long ab = idxmap[k];
+    double dpq = -INFINITY;
+    for (long long i = 0; i < vd_dim[ab]; ++i) dpq = std::max(dpq, vdiag[vd_loc[ab] - 1 + i]);
+    if (dpq <= pthresh) k0 = k;   // the last discarded pair stays in BASE
+    else break;
+  }
+  const long long nsp = max_sp - k0 + 1;
+  long long nop = 0;
+  for (long long k = k0; k <= max_sp; ++k) nop += vd_dim[idxmap[k]];
+  // 1-based BASE tables
+  std::vector<int> spA(nsp + 1), spB(nsp + 1);
+  std::vector<long long> spdim(nsp + 1), loc_full(nsp + 1), loc_sub(nsp + 1, 0), idx_sub(nsp + 1, 0),
+      loc_comp(nsp + 1);
+  std::vector<double> Dh(nop);
+  {
+    long long s = 1, p = 1;
+    for (long long k = k0; k <= max_sp; ++k) {
+      long long l = idxmap[k];
+      spA[s] = vd_a[l];
+      spB[s] = vd_b[l];
+      spdim[s] = vd_dim[l];
+      loc_full[s] = p;
+      for (long long i = 0; i < vd_dim[l]; ++i) Dh[p - 1 + i] = vdiag[vd_loc[l] - 1 + i];
+      ++s;
+      p += vd_dim[l];
+    }
+  }
+  S.nshell_pairs = nsp;
+  S.norb_pairs = nop;
+
+  // ---------------- chol2_DefineSubsets / DivideOrbPairs (one image)
+  const long long nsub = (long long)std::ceil((double)nop / (double)max_block_dim);
+  std::vector<long long> sdim(nsub + 1, 0), sb0(nsub + 1, 0), sb1(nsub + 1, 0);
+  {
+    long long target = (long long)std::ceil((double)nop / (double)nsub);
+    long long berr[2] = {0, 0}, bloc[2] = {0, 0};
+    for (long long k = 1; k <= nsub; ++k) {
+      if (k == 1) sb0[k] = 1;
+      else sb0[k] = (sb1[k - 1] < nsp && sb1[k - 1] > 0) ? sb1[k - 1] + 1 : 0;
+      sdim[k] = 0;
+      sb1[k] = sb0[k];
+      if (sb0[k] > 0) {
+        if (k < nsub) {
+          berr[0] = nop;
+          berr[1] = nop;
+          for (long long s = sb0[k]; s <= nsp; ++s) {
+            loc_sub[s] = sdim[k] + 1;
+            idx_sub[s] = k;
+            sdim[k] += spdim[s];
+            if (sdim[k] >= target) { bloc[1] = s; berr[1] = sdim[k] - target; break; }
+            bloc[0] = s;
+            berr[0] = sdim[k] - target;
+          }
+          if (std::llabs(berr[0]) < std::llabs(berr[1])) { sb1[k] = bloc[0]; sdim[k] = berr[0] + target; }
+          else { sb1[k] = bloc[1]; sdim[k] = berr[1] + target; }
+        } else {
+          sb1[k] = nsp;
+          for (long long s = sb0[k]; s <= nsp; ++s) {
+            loc_sub[s] = sdim[k] + 1;
+            idx_sub[s] = k;
+            sdim[k] += spdim[s];
+          }
+        }
+      }
+    }
+  }
+  long long msd = 0;
+  for (long long k = 1; k <= nsub; ++k) msd = std::max(msd, sdim[k]);
+  S.nsubsets = nsub;
+  S.max_subset_dim = msd;
+  const long long max_nchol = (long long)std::min(std::ceil((double)nao * 15.0), (double)nop);
+
+  // ---------------- chol2_Pivots
+  std::vector<long long> pivots(nop + 1, 0), pivot_order;
+  long long nchol = 0;
+  bool converged = false;
+  double dmax = 0.0;
+  double t_eri1 = 0, t_rtr = 0, t_micro = 0;
+  {
+    double t0 = now();
+    long long nsps_cand = nsp, nops_cand = nop;
+    std::vector<long long> cand(nsp + 1);   // candidate shell pairs, compressed order
+    for (long long k = 1; k <= nsp; ++k) { cand[k] = k; loc_comp[k] = loc_full[k]; }
+    DBuf<double> dD, dDn, dL, dLn, dM, dLB, dtmp, dbm;
+    dD.upload(Dh.data(), nop, C.stream);
+    long long ncand = nop;       // rows of L / length of D
+    long long capL = 0;          // columns allocated in L
+    dbm.alloc(2);
+    dmax = -INFINITY;
+    for (double v : Dh) dmax = std::max(dmax, v);
+    bool do_macro = max_nchol > 0;
+    DBuf<long long> dsrc, dloc, dbd;
+    DBuf<int> ddim;
+    std::vector<double> mv;
+    std::vector<long long> im;
+    while (do_macro) {
+      S.nmacro++;
+      // chol2_DefineCandidates: max D per candidate pair, descending sort
+      long long n = nsps_cand;
+      mv.assign(n + 1, 0.0);
+      im.assign(n + 1, 0);
+      if (n > 0) {
+        std::vector<long long> hl(n);
+        std::vector<int> hd(n);
+        for (long long k = 1; k <= n; ++k) { hl[k - 1] = loc_comp[cand[k]] - 1; hd[k - 1] = (int)spdim[cand[k]]; }
+        dloc.upload(hl, C.stream);
+        ddim.upload(hd, C.stream);
+        dtmp.alloc(n);
+        hipLaunchKernelGGL(k_segmax, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, C.stream, dD.p, dloc.p,
+                           ddim.p, n, dtmp.p);
+        GI_HIP(hipMemcpyAsync(mv.data() + 1, dtmp.p, sizeof(double) * n, hipMemcpyDeviceToHost, C.stream));
+        GI_HIP(hipStreamSynchronize(C.stream));
+        for (long long k = 1; k <= n; ++k) {
+          im[k] = cand[k];
+          if (!raw_ties) mv[k] = qabs(mv[k], inv_delta) / inv_delta;
+        }
+      }
+      dsort0(mv.data() + 1, im.data() + 1, n, -2);
+      nsps_cand = 0;
+      nops_cand = 0;
+      for (long long k = 1; k <= n; ++k) {
+        if (mv[k] > tau) {
+          ++nsps_cand;
+          cand[k] = im[k];
+          nops_cand += spdim[im[k]];
+        } else break;
+      }
+      // chol2_Compress: move D and the stored vectors to the new candidate order
+      long long max_nbatch_cap = std::max<long long>((long long)maxnang * maxnang, 1000);
+      if (nsps_cand > 0) {
+        std::vector<long long> src;
+        src.reserve(nops_cand);
+        std::vector<long long> newloc(nsps_cand + 1);
+        long long ncomp = 0;
+        for (long long g = 1; g <= nsps_cand; ++g) {
+          long long ab = cand[g];
+          newloc[g] = ncomp + 1;
+          for (long long i = 0; i < spdim[ab]; ++i) src.push_back(loc_comp[ab] - 1 + i);
+          ncomp += spdim[ab];
+        }
+        dsrc.upload(src, C.stream);
+        dDn.alloc(ncomp);
+        hipLaunchKernelGGL(k_gather_rows_of_cols, dim3((unsigned)((ncomp + 255) / 256), 1), dim3(256), 0, C.stream,
+                           dDn.p, ncomp, dD.p, ncand, dsrc.p, ncomp, 1LL);
+        std::swap(dD.p, dDn.p);
+        std::swap(dD.n, dDn.n);
+        // new L with room for this macro iteration's vectors
+        long long mnb = std::min(ncomp, max_nbatch_cap);
+        long long newcap = std::min(max_nchol, nchol + mnb);
+        newcap = std::max(newcap, (long long)1);
+        dLn.alloc((size_t)ncomp * newcap);
+        if (nchol > 0) {
+          unsigned gy = (unsigned)std::min<long long>(nchol, 65535);
+          hipLaunchKernelGGL(k_gather_rows_of_cols, dim3((unsigned)((ncomp + 255) / 256), gy), dim3(256), 0,
+                             C.stream, dLn.p, ncomp, dL.p, ncand, dsrc.p, ncomp, nchol);
+        }
+        std::swap(dL.p, dLn.p);
+        std::swap(dL.n, dLn.n);
+        dLn.release();
+        capL = newcap;
+        for (long long g = 1; g <= nsps_cand; ++g) loc_comp[cand[g]] = newloc[g];
+        ncand = ncomp;
+      }
+      const long long max_nbatch = std::min(nops_cand, max_nbatch_cap);
+      double dmin = std::max(0.01 * dmax, tau);
+      // chol2_DefineBatch: candidates are already in descending order of their
+      // maxima; the reference sorts them again (ties may be permuted)
+      std::vector<double> ds(nsps_cand + 1);
+      std::vector<long long> di(nsps_cand + 1);
+      for (long long k = 1; k <= nsps_cand; ++k) { ds[k] = mv[k]; di[k] = cand[k]; }
+      dsort0(ds.data() + 1, di.data() + 1, nsps_cand, -2);
+      long long nsps_batch = 0, nops_batch = 0;
+      std::vector<long long> bsp(1), colloc(1);
+      for (long long k = 1; k <= nsps_cand; ++k) {
+        long long ab = di[k];
+        if (ds[k] > dmin && nops_batch + spdim[ab] <= max_nbatch) {
+          bsp.push_back(ab);
+          colloc.push_back(nops_batch + 1);
+          nsps_batch = k;
+          nops_batch += spdim[ab];
+        } else break;
+      }
+      // batch element tables: compressed row, FULL location
+      std::vector<long long> bd(nops_batch), bfull(nops_batch);
+      for (long long k = 1; k <= nsps_batch; ++k)
+        for (long long p = 0; p < spdim[bsp[k]]; ++p) {
+          bd[colloc[k] - 1 + p] = loc_comp[bsp[k]] - 1 + p;
+          bfull[colloc[k] - 1 + p] = loc_full[bsp[k]] + p;
+        }
+      dbd.upload(bd, C.stream);
+      // chol2_M: ERIs (candidates | batch)
+      dM.alloc((size_t)std::max<long long>(ncand, 1) * std::max<long long>(max_nbatch, 1));
+      {
+        double te = now();
+        EriPhaseTimer pt(C, "step1");
+        std::vector<PairDesc> rd(nsps_cand), cd(nsps_batch);
+        std::vector<int> rows(nsps_cand), cols(nsps_batch);
+        for (long long k = 1; k <= nsps_cand; ++k) {
+          long long ab = cand[k];
+          rd[k - 1] = PairDesc{spA[ab], spB[ab], spA[ab] == spB[ab] ? PAIR_SAME : 0, 0, 0, 0, loc_comp[ab] - 1};
+          rows[k - 1] = (int)(k - 1);
+        }
+        for (long long k = 1; k <= nsps_batch; ++k) {
+          long long ab = bsp[k];
+          cd[k - 1] = PairDesc{spA[ab], spB[ab], spA[ab] == spB[ab] ? PAIR_SAME : 0, 0, 0, 0, ncand * (colloc[k] - 1)};
+          cols[k - 1] = (int)(k - 1);
+        }
+        if (nsps_batch > 0 && nsps_cand > 0) {
+          DBuf<PairDesc> drd, dcd;
+          drd.upload(rd, C.stream);
+          dcd.upload(cd, C.stream);
+          EriRequest R;
+          R.rdesc = &rd; R.cdesc = &cd; R.d_rdesc = drd.p; R.d_cdesc = dcd.p;
+          R.rows = &rows; R.cols = &cols; R.mode = LAUNCH_PRODUCT;
+          R.out = dM.p; R.ldo = ncand; R.kappa = kappa;
+          eri_run(C, R);
+        }
+        t_eri1 += now() - te;
+      }
+      // chol2_SubtractR: M(:, batch) -= L^T L(batch)  (vectors are columns of L)
+      if (nchol > 0 && nops_batch > 0) {
+        double tr = now();
+        dLB.alloc((size_t)nops_batch * nchol);
+        unsigned gy = (unsigned)std::min<long long>(nchol, 65535);
+        hipLaunchKernelGGL(k_gather_rows_of_cols, dim3((unsigned)((nops_batch + 255) / 256), gy), dim3(256), 0,
+                           C.stream, dLB.p, nops_batch, dL.p, ncand, dbd.p, nops_batch, nchol);
+        const double alpha = -1.0, beta = 1.0;
+        ROCBLAS_CHECK(rocblas_dgemm(C.blas, rocblas_operation_none, rocblas_operation_transpose, ncand, nops_batch,
+                                    nchol, &alpha, dL.p, ncand, dLB.p, nops_batch, &beta, dM.p, ncand));
+        GI_HIP(hipStreamSynchronize(C.stream));
+        t_rtr += now() - tr;
+      }
+      // micro iterations
+      double tm = now();
+      double bmh[2] = {-1.0, -1.0};
+      auto batchmax = [&]() {
+        if (nops_batch == 0) { bmh[0] = -1.0; bmh[1] = -1.0; return; }
+        hipLaunchKernelGGL(k_batchmax, dim3(1), dim3(1024), 0, C.stream, dD.p, dbd.p, nops_batch,
+                           raw_ties ? 0.0 : inv_delta, dbm.p);
+        GI_HIP(hipMemcpyAsync(bmh, dbm.p, sizeof(double) * 2, hipMemcpyDeviceToHost, C.stream));
+        GI_HIP(hipStreamSynchronize(C.stream));
+      };
+      batchmax();
+      const long long j0 = nchol;
+      long long j = 0;
+      bool do_micro = nchol < max_nchol && nops_batch > 0;
+      while (do_micro) {
+        if (bmh[1] < 0.0) break;   // no batch element above -1: nothing to pivot on
+        ++j;
+        ++nchol;
+        const long long e = (long long)bmh[1];
+        pivots[bfull[e]] = 1;
+        pivot_order.push_back(bfull[e]);
+        hipLaunchKernelGGL(k_lj, dim3((unsigned)((ncand + 255) / 256)), dim3(256), 0, C.stream, dM.p, ncand, dL.p,
+                           ncand, j0, nchol - 1, dbd.p, dbm.p, dD.p);
+        batchmax();
+        if (bmh[0] <= dmin) do_micro = false;
+        if (j == nops_batch || nchol == max_nchol) do_micro = false;
+      }
+      t_micro += now() - tm;
+      dmax = (ncand > 0) ? device_max(C, dD.p, ncand, dtmp) : 0.0;
+      if (nsps_cand == 0) dmax = 0.0;
+      if (dmax <= tau) { converged = true; do_macro = false; }
+      if (nchol == max_nchol || nops_batch == 0) do_macro = false;
+    }
+    S.timings["step1"] = now() - t0;
+  }
+  S.timings["eri_step1"] = t_eri1;
+  S.timings["rtr"] = t_rtr;
+  S.timings["micro"] = t_micro;
+  S.max_residual = dmax;
+  S.nvecs = nchol;
+  // step 1 outputs
+  S.shell_pairs.assign(2 * nsp, 0);
+  S.shell_pair_loc.assign(3 * nsp, 0);
+  S.shell_pair_dim.assign(nsp, 0);
+  for (long long s = 1; s <= nsp; ++s) {
+    S.shell_pairs[2 * (s - 1)] = spA[s] + 1;
+    S.shell_pairs[2 * (s - 1) + 1] = spB[s] + 1;
+    S.shell_pair_loc[3 * (s - 1)] = loc_full[s];
+    S.shell_pair_loc[3 * (s - 1) + 1] = loc_sub[s];
+    S.shell_pair_loc[3 * (s - 1) + 2] = idx_sub[s];
+    S.shell_pair_dim[s - 1] = spdim[s];
+  }
+  S.subset_dim.assign(nsub, 0);
+  S.subset_bounds.assign(2 * nsub, 0);
+  for (long long k = 1; k <= nsub; ++k) {
+    S.subset_dim[k - 1] = sdim[k];
+    S.subset_bounds[2 * (k - 1)] = sb0[k];
+    S.subset_bounds[2 * (k - 1) + 1] = sb1[k];
+  }
+  S.pivots.assign(pivots.begin() + 1, pivots.end());
+  S.pivot_order = pivot_order;
+  if (nchol == max_nchol && !converged)
+    throw Error{20, "Max number of Cholesky vectors reached without convergence"};
+
+  // ---------------- chol2_Step2
+  const long long nvec = nchol;
+  double t2 = now();
+  std::vector<long long> psp, psploc, pspdim, porb;
+  {
+    long long nsum = 0;
+    for (long long i = 1; i <= nop; ++i) nsum += pivots[i];
+    if (nsum != nvec) throw Error{30, "Invalid Pivots array"};
+    long long q = 0;
+    for (long long ab = 1; ab <= nsp; ++ab) {
+      long long p0 = loc_full[ab], p1 = p0 + spdim[ab] - 1, s = 0;
+      for (long long p = p0; p <= p1; ++p) s += pivots[p];
+      if (s == 0) continue;
+      psp.push_back(ab);
+      psploc.push_back(q + 1);
+      pspdim.push_back(s);
+      if (spA[ab] != spB[ab]) {
+        for (long long p = p0; p <= p1; ++p)
+          if (pivots[p] == 1) { porb.push_back(p - p0 + 1); ++q; }
+      } else {
+        long long na = C.shell_nang[spA[ab]];
+        for (long long p = p0; p <= p1; ++p)
+          if (pivots[p] == 1) {
+            long long pos = p - p0 + 1, a = 0, b = 0;
+            if (flags & 1) {
+              long long cnt = 0;
+              for (long long y = 1; y <= na && !a; ++y)
+                for (long long x = y; x <= na; ++x)
+                  if (++cnt == pos) { a = x; b = y; break; }
+            } else {
+              // chol2_pq2p_ge_q, TwoStepCholesky_Step2.f90:422-466
+              long long qb = (pos - 1) / (na + 1) + 1;
+              long long v = pos - (na + 1) * (qb - 1);
+              long long i1 = na - qb + 1;
+              long long in2 = v / (i1 + 1), in1 = in2 ^ 1;
+              a = in1 * (qb + v - 1) + in2 * (v - i1 + na - qb);
+              b = in1 * qb + in2 * i1;
+            }
+            porb.push_back(a + (b - 1) * na);
+            ++q;
+          }
+      }
+    }
+  }
+  const long long npsp = (long long)psp.size();
+  S.npivot_shell_pairs = npsp;
+  S.pivot_shell_pairs = psp;
+  S.pivot_shell_pair_loc = psploc;
+  S.pivot_shell_pair_dim = pspdim;
+  S.pivot_orb_pairs = porb;
+  // selection list: 0-based full-block indices of the pivots
+  std::vector<int> sel(nvec);
+  for (long long i = 0; i < nvec; ++i) sel[i] = (int)(porb[i] - 1);
+  DBuf<int> dsel;
+  dsel.upload(sel, C.stream);
+  // chol2_Vpqrs
+  S.inv_l.alloc((size_t)nvec * nvec);
+  {
+    double tv = now();
+    EriPhaseTimer pt(C, "vpqrs");
+    std::vector<PairDesc> rd(npsp), cd(npsp);
+    std::vector<int> rows(npsp);
+    for (long long k = 0; k < npsp; ++k) {
+      long long ab = psp[k];
+      rd[k] = PairDesc{spA[ab], spB[ab], PAIR_SEL, (int)(psploc[k] - 1), (int)pspdim[k], 0, psploc[k] - 1};
+      cd[k] = PairDesc{spA[ab], spB[ab], PAIR_SEL, (int)(psploc[k] - 1), (int)pspdim[k], 0, nvec * (psploc[k] - 1)};
+      rows[k] = (int)k;
+    }
+    DBuf<PairDesc> drd, dcd;
+    drd.upload(rd, C.stream);
+    dcd.upload(cd, C.stream);
+    EriRequest R;
+    R.rdesc = &rd; R.cdesc = &cd; R.d_rdesc = drd.p; R.d_cdesc = dcd.p;
+    R.rows = &rows; R.cols = &rows; R.mode = LAUNCH_PRODUCT;
+    R.out = S.inv_l.p; R.ldo = nvec; R.d_sel = dsel.p; R.kappa = kappa;
+    eri_run(C, R);
+    S.timings["vpqrs"] = now() - tv;
+  }
+  // V = L L^T, L^-1 (real_Cholesky + real_LowerTriangularInverse)
+  {
+    double tc = now();
+    DBuf<rocblas_int> dinfo;
+    dinfo.alloc(1);
+    ROCBLAS_CHECK(rocsolver_dpotrf(C.blas, rocblas_fill_lower, (rocblas_int)nvec, S.inv_l.p, (rocblas_int)nvec,
+                                   dinfo.p));
+    rocblas_int info = 0;
+    GI_HIP(hipMemcpyAsync(&info, dinfo.p, sizeof(info), hipMemcpyDeviceToHost, C.stream));
+    GI_HIP(hipStreamSynchronize(C.stream));
+    if (info != 0) throw Error{40, "Cholesky factorization failed with exit code info=" + std::to_string(info)};
+    hipLaunchKernelGGL(k_zero_upper, dim3((unsigned)((nvec + 255) / 256), (unsigned)nvec), dim3(256), 0, C.stream,
+                       S.inv_l.p, nvec);
+    ROCBLAS_CHECK(rocsolver_dtrtri(C.blas, rocblas_fill_lower, rocblas_diagonal_non_unit, (rocblas_int)nvec,
+                                   S.inv_l.p, (rocblas_int)nvec, dinfo.p));
+    GI_HIP(hipMemcpyAsync(&info, dinfo.p, sizeof(info), hipMemcpyDeviceToHost, C.stream));
+    GI_HIP(hipStreamSynchronize(C.stream));
+    if (info != 0) throw Error{41, "Lower triangular inverse failed with exit code info=" + std::to_string(info)};
+    S.timings["potrf_trtri"] = now() - tc;
+  }
+  S.timings["step2"] = now() - t2;
+
+  // ---------------- full-dimension vectors, Rkpq(:,:,X) = Inv_L Vabrs^T
+  if (!(flags & 2)) {
+    double tf = now();
+    const size_t slab = (size_t)msd * nvec;
+    S.rkpq.alloc((size_t)nvec * msd * nsub);
+    // subsets per group: keep the Vabrs staging under ~16 GiB
+    long long per_group = std::max<long long>(1, (long long)((16.0 * (1ull << 30)) / (8.0 * (double)std::max<size_t>(slab, 1))));
+    per_group = std::min(per_group, nsub);
+    DBuf<double> dW;
+    dW.alloc(slab * per_group);
+    std::vector<PairDesc> cd(npsp);
+    std::vector<int> cols(npsp);
+    for (long long k = 0; k < npsp; ++k) {
+      long long ab = psp[k];
+      cd[k] = PairDesc{spA[ab], spB[ab], PAIR_SEL, (int)(psploc[k] - 1), (int)pspdim[k], 0, msd * (psploc[k] - 1)};
+      cols[k] = (int)k;
+    }
+    DBuf<PairDesc> dcd;
+    dcd.upload(cd, C.stream);
+    double t_vab = 0, t_gemm = 0;
+    for (long long X0 = 1; X0 <= nsub; X0 += per_group) {
+      long long X1 = std::min(nsub, X0 + per_group - 1);
+      long long ng = X1 - X0 + 1;
+      GI_HIP(hipMemsetAsync(dW.p, 0, sizeof(double) * slab * ng, C.stream));
+      std::vector<PairDesc> rd;
+      std::vector<int> rows;
+      for (long long X = X0; X <= X1; ++X) {
+        if (sb0[X] <= 0) continue;
+        for (long long s = sb0[X]; s <= sb1[X]; ++s) {
+          rd.push_back(PairDesc{spA[s], spB[s], spA[s] == spB[s] ? PAIR_SAME : 0, 0, 0, 0,
+                                (long long)((X - X0) * slab) + loc_sub[s] - 1});
+          rows.push_back((int)rd.size() - 1);
+        }
+      }
+      double tv = now();
+      if (!rd.empty() && npsp > 0) {
+        EriPhaseTimer pt(C, "vabrs");
+        DBuf<PairDesc> drd;
+        drd.upload(rd, C.stream);
+        EriRequest R;
+        R.rdesc = &rd; R.cdesc = &cd; R.d_rdesc = drd.p; R.d_cdesc = dcd.p;
+        R.rows = &rows; R.cols = &cols; R.mode = LAUNCH_PRODUCT;
+        R.out = dW.p; R.ldo = msd; R.d_sel = dsel.p; R.kappa = kappa;
+        eri_run(C, R);
+      }
+      t_vab += now() - tv;
+      double tg = now();
+      const double one = 1.0, zero = 0.0;
+      ROCBLAS_CHECK(rocblas_dgemm_strided_batched(
+          C.blas, rocblas_operation_none, rocblas_operation_transpose, nvec, msd, nvec, &one, S.inv_l.p, nvec, 0,
+          dW.p, msd, (rocblas_stride)slab, &zero, S.rkpq.p + (size_t)(X0 - 1) * nvec * msd, nvec,
+          (rocblas_stride)((size_t)nvec * msd), ng));
+      GI_HIP(hipStreamSynchronize(C.stream));
+      t_gemm += now() - tg;
+    }
+    S.timings["vabrs"] = t_vab;
+    S.timings["vectors_gemm"] = t_gemm;
+    S.timings["vectors"] = now() - tf;
+  }
+  GI_HIP(hipStreamSynchronize(C.stream));
+  S.timings["total"] = now() - t_begin;
+  S.nquartets = C.eri_quartets_acc;
+  S.eri_flops = C.eri_flops_acc;
+  S.timings["eri_kernel"] = C.eri_ms_acc / 1000.0;
+  S.valid = true;
+}
+
+}  // namespace gi

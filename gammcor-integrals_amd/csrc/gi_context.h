@@ -1,0 +1,110 @@
+// Library context: one per (process, device).  Owns the device copy of the
+// basis, the Boys table, the ERI workspace and the Cholesky results.
+#pragma once
+#include <rocblas/rocblas.h>
+#include <map>
+#include <memory>
+#include <vector>
+#include "eri_kernel.h"
+#include "gi_common.h"
+
+namespace gi {
+
+// owning device buffer
+template <class T>
+struct DBuf {
+  T *p = nullptr;
+  size_t n = 0;
+  DBuf() = default;
+  DBuf(const DBuf &) = delete;
+  DBuf &operator=(const DBuf &) = delete;
+  ~DBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  void alloc(size_t count) {
+    if (count <= n && p) return;
+    release();
+    if (count == 0) count = 1;
+    GI_HIP(hipMalloc((void **)&p, count * sizeof(T)));
+    n = count;
+  }
+  void upload(const T *src, size_t count, hipStream_t s) {
+    alloc(count);
+    if (count) GI_HIP(hipMemcpyAsync(p, src, count * sizeof(T), hipMemcpyHostToDevice, s));
+  }
+  void upload(const std::vector<T> &v, hipStream_t s) { upload(v.data(), v.size(), s); }
+};
+
+struct ClassPlan {
+  int mode, G, qpw, per_q;
+  long long flops;  // FP64 operations per spherical quartet
+};
+const ClassPlan *class_plan(int la, int lb, int lc, int ld);   // canonical class
+bool class_supported(int la, int lb, int lc, int ld);
+
+// Cholesky outputs kept on the device / host (TChol2Vecs + Rkpq)
+struct CholState {
+  bool valid = false;
+  double tau = 0, kappa = 0;
+  long long nvecs = 0, norb_pairs = 0, nshell_pairs = 0, nsubsets = 0, max_subset_dim = 0;
+  long long npivot_shell_pairs = 0, nmacro = 0, nquartets = 0;
+  double max_residual = 0;
+  std::vector<long long> shell_pairs, shell_pair_loc, shell_pair_dim, subset_dim, subset_bounds,
+      pivots, pivot_order, pivot_shell_pairs, pivot_shell_pair_loc, pivot_shell_pair_dim,
+      pivot_orb_pairs;
+  std::vector<double> vdiag;
+  DBuf<double> inv_l;   // nvecs x nvecs
+  DBuf<double> rkpq;    // nvecs x max_subset_dim x nsubsets
+  DBuf<long long> mo_map;   // AO pair (p + nao q) -> Rkpq column, -1 if screened
+  std::map<std::string, double> timings;
+  std::map<std::string, double> eri_kernel_ms;   // per phase ERI kernel time
+  std::map<std::string, long long> eri_quartets; // per phase quartet counts
+  long long eri_flops = 0;
+};
+
+struct Context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  rocblas_handle blas = nullptr;
+  // basis
+  int natoms = 0, nshells = 0, spher = 1, lmax = 0, nao = 0;
+  std::vector<DevShell> hshells;
+  std::vector<int> shell_l, shell_nang, shell_np, shell_loc;   // shell_loc: 0-based first AO
+  DBuf<DevShell> dshells;
+  DBuf<double> dexps, dcntr, dnorms, dboys;
+  DBuf<double> gws;
+  bool basis_set = false;
+  CholState chol;
+  // profiling of ERI launches (events on the library stream)
+  bool time_eri = false;
+  double eri_ms_acc = 0;
+  long long eri_quartets_acc = 0;
+  long long eri_flops_acc = 0;
+
+  Context(int dev);
+  ~Context();
+};
+
+// ---------------------------------------------------------------- ERI driver
+// Evaluate the quartets of a row-list x column-list product (or zipped lists)
+// into a column-major destination, binning pairs by angular momenta.
+struct EriRequest {
+  const std::vector<PairDesc> *rdesc = nullptr;   // host copies (for binning)
+  const std::vector<PairDesc> *cdesc = nullptr;
+  const PairDesc *d_rdesc = nullptr, *d_cdesc = nullptr;
+  const std::vector<int> *rows = nullptr, *cols = nullptr;
+  int mode = LAUNCH_PRODUCT;
+  double *out = nullptr;
+  long long ldo = 0;
+  const int *d_sel = nullptr;
+  double kappa = 0;
+};
+void eri_run(Context &C, const EriRequest &R);
+
+// one-time device tables
+void upload_constants(Context &C);
+
+}  // namespace gi

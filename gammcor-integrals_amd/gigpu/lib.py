@@ -1,0 +1,211 @@
+"""ctypes binding of libgigpu (C ABI declared in include/gigpu.h).
+
+The binding loads the in-tree ``gigpu/libgigpu.so``.  There is no fallback:
+if the library is missing or no GPU is present, calls fail loudly.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libgigpu.so")
+
+EXPORTS = ["gi_create", "gi_destroy", "gi_last_error", "gi_set_basis", "gi_eri_quartets",
+           "gi_chol_build", "gi_chol_accuracy_tau", "gi_chol_dims", "gi_chol_meta", "gi_chol_rkpq",
+           "gi_chol_rkpq_device", "gi_chol_inv_l", "gi_chol_vdiag", "gi_chol_timings",
+           "gi_mo_transform", "gi_mo_transform_device"]
+
+CHOL_DECODE_FIXED = 1
+CHOL_SKIP_VECTORS = 2
+CHOL_TIME_ERI = 4
+CHOL_RAW_TIES = 8
+MO_ALL_COLUMNS = 1
+
+i64 = C.c_int64
+dptr = C.POINTER(C.c_double)
+iptr = C.POINTER(C.c_int64)
+
+_lib = None
+
+
+class GigpuError(RuntimeError):
+    pass
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise GigpuError("libgigpu.so is not built (%s); run `python -m gigpu.build`" % LIB_PATH)
+    L = C.CDLL(LIB_PATH)
+    L.gi_last_error.restype = C.c_char_p
+    L.gi_create.argtypes = [C.POINTER(C.c_void_p), i64]
+    L.gi_destroy.argtypes = [C.c_void_p]
+    L.gi_set_basis.argtypes = [C.c_void_p, i64, dptr, i64, iptr, iptr, i64, iptr, iptr, i64, dptr, dptr, i64,
+                               dptr, i64]
+    L.gi_eri_quartets.argtypes = [C.c_void_p, i64, iptr, C.c_double, iptr, dptr, i64]
+    L.gi_chol_build.argtypes = [C.c_void_p, C.c_double, C.c_double, i64, i64, i64]
+    L.gi_chol_accuracy_tau.argtypes = [i64, dptr]
+    L.gi_chol_dims.argtypes = [C.c_void_p, iptr]
+    L.gi_chol_meta.argtypes = [C.c_void_p] + [iptr] * 11
+    L.gi_chol_rkpq.argtypes = [C.c_void_p, dptr]
+    L.gi_chol_rkpq_device.argtypes = [C.c_void_p, C.POINTER(C.c_void_p)]
+    L.gi_chol_inv_l.argtypes = [C.c_void_p, dptr]
+    L.gi_chol_vdiag.argtypes = [C.c_void_p, dptr]
+    L.gi_chol_timings.argtypes = [C.c_void_p, dptr]
+    L.gi_mo_transform.argtypes = [C.c_void_p, dptr, i64, i64, i64, dptr, i64, i64, i64, i64, i64, dptr]
+    L.gi_mo_transform_device.argtypes = [C.c_void_p, C.c_void_p, i64, i64, i64, C.c_void_p, i64, i64, i64, i64,
+                                         i64, C.c_void_p]
+    for name in EXPORTS:
+        getattr(L, name).restype = C.c_int if name != "gi_last_error" else C.c_char_p
+    _lib = L
+    return L
+
+
+def _check(status):
+    if status != 0:
+        msg = _lib.gi_last_error().decode(errors="replace")
+        raise GigpuError("gigpu error %d: %s" % (status, msg))
+
+
+def _p(a, t=C.c_double):
+    return a.ctypes.data_as(C.POINTER(t))
+
+
+class Context:
+    """One libgigpu context bound to one HIP device."""
+
+    def __init__(self, device: int = 0):
+        self.L = load()
+        h = C.c_void_p()
+        _check(self.L.gi_create(C.byref(h), device))
+        self.h = h
+        self.basis = None
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.gi_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- basis -----------------------------------------------------------
+    def set_basis(self, basis):
+        a = basis.c_arrays()
+        self._basis_arrays = a
+        _check(self.L.gi_set_basis(self.h, basis.natoms, _p(a["coords"]), basis.nshells,
+                                   _p(a["shell_centers"], i64), _p(a["shell_params"], i64), basis.nparams,
+                                   _p(a["momentum"], i64), _p(a["nprim"], i64), a["cntr"].shape[1],
+                                   _p(a["cntr"]), _p(a["exps"]), a["norm"].shape[1], _p(a["norm"]),
+                                   1 if basis.spher_ao else 0))
+        self.basis = basis
+
+    # -- ERIs ------------------------------------------------------------
+    def eri_quartets(self, quartets, kappa=0.0):
+        """List of V[a,b,c,d] blocks for 0-based shell quartets."""
+        q = np.ascontiguousarray(np.asarray(quartets, dtype=np.int64).reshape(-1, 4))
+        dims = [[self.basis.shell_nang(int(s)) for s in row] for row in q]
+        sizes = np.array([d[0] * d[1] * d[2] * d[3] for d in dims], dtype=np.int64)
+        offs = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64)
+        out = np.zeros(int(sizes.sum()) if len(sizes) else 1)
+        _check(self.L.gi_eri_quartets(self.h, len(q), _p(q, i64), kappa, _p(offs, i64), _p(out), len(out)))
+        blocks = []
+        for (na, nb, nc, nd), o, n in zip(dims, offs, sizes):
+            blocks.append(out[o:o + n].reshape(nd, nc, nb, na).transpose(3, 2, 1, 0))
+        return blocks
+
+    # -- Cholesky --------------------------------------------------------
+    def chol_build(self, tau=1e-6, kappa=0.0, max_block_dim=4000, vec_block=500, flags=0):
+        _check(self.L.gi_chol_build(self.h, tau, kappa, max_block_dim, vec_block, flags))
+
+    def chol_dims(self):
+        d = np.zeros(8, np.int64)
+        _check(self.L.gi_chol_dims(self.h, _p(d, i64)))
+        return dict(nvecs=int(d[0]), norb_pairs=int(d[1]), nshell_pairs=int(d[2]), nsubsets=int(d[3]),
+                    max_subset_dim=int(d[4]), npivot_shell_pairs=int(d[5]), nmacro=int(d[6]),
+                    nquartets=int(d[7]))
+
+    def chol_meta(self):
+        d = self.chol_dims()
+        nsp, nsub, nop, nv, npsp = (d["nshell_pairs"], d["nsubsets"], d["norb_pairs"], d["nvecs"],
+                                    d["npivot_shell_pairs"])
+        arrs = dict(shell_pairs=np.zeros(2 * nsp, np.int64), shell_pair_loc=np.zeros(3 * nsp, np.int64),
+                    shell_pair_dim=np.zeros(nsp, np.int64), subset_dim=np.zeros(nsub, np.int64),
+                    subset_bounds=np.zeros(2 * nsub, np.int64), pivots=np.zeros(nop, np.int64),
+                    pivot_order=np.zeros(max(nv, 1), np.int64), pivot_shell_pairs=np.zeros(max(npsp, 1), np.int64),
+                    pivot_shell_pair_loc=np.zeros(max(npsp, 1), np.int64),
+                    pivot_shell_pair_dim=np.zeros(max(npsp, 1), np.int64),
+                    pivot_orb_pairs=np.zeros(max(nv, 1), np.int64))
+        keys = ["shell_pairs", "shell_pair_loc", "shell_pair_dim", "subset_dim", "subset_bounds", "pivots",
+                "pivot_order", "pivot_shell_pairs", "pivot_shell_pair_loc", "pivot_shell_pair_dim",
+                "pivot_orb_pairs"]
+        _check(self.L.gi_chol_meta(self.h, *[_p(arrs[k], i64) for k in keys]))
+        arrs["shell_pairs"] = arrs["shell_pairs"].reshape(nsp, 2)
+        arrs["shell_pair_loc"] = arrs["shell_pair_loc"].reshape(nsp, 3)
+        arrs["subset_bounds"] = arrs["subset_bounds"].reshape(nsub, 2)
+        arrs["pivot_order"] = arrs["pivot_order"][:nv]
+        arrs["pivot_orb_pairs"] = arrs["pivot_orb_pairs"][:nv]
+        for k in ("pivot_shell_pairs", "pivot_shell_pair_loc", "pivot_shell_pair_dim"):
+            arrs[k] = arrs[k][:npsp]
+        arrs.update(d)
+        return arrs
+
+    def chol_rkpq(self):
+        d = self.chol_dims()
+        out = np.zeros(d["nvecs"] * d["max_subset_dim"] * d["nsubsets"])
+        _check(self.L.gi_chol_rkpq(self.h, _p(out)))
+        return out.reshape(d["nsubsets"], d["max_subset_dim"], d["nvecs"])
+
+    def chol_rkpq_device_ptr(self):
+        p = C.c_void_p()
+        _check(self.L.gi_chol_rkpq_device(self.h, C.byref(p)))
+        return p.value
+
+    def chol_inv_l(self):
+        n = self.chol_dims()["nvecs"]
+        out = np.zeros(n * n)
+        _check(self.L.gi_chol_inv_l(self.h, _p(out)))
+        return out.reshape(n, n).T
+
+    def chol_vdiag(self):
+        n = self.basis.nao
+        out = np.zeros(n * (n + 1) // 2)
+        _check(self.L.gi_chol_vdiag(self.h, _p(out)))
+        return out
+
+    def chol_timings(self):
+        t = np.zeros(14)
+        _check(self.L.gi_chol_timings(self.h, _p(t)))
+        keys = ["step1", "step2", "vectors", "total", "eri_kernel", "eri_flops", "vdiag", "eri_step1", "rtr",
+                "micro", "vpqrs", "potrf_trtri", "vabrs", "vectors_gemm"]
+        return dict(zip(keys, t.tolist()))
+
+    def mo_transform(self, CA, a0, a1, CB, b0, b1, max_buffer_mb=1 << 20, flags=0):
+        CA = np.asfortranarray(CA, dtype=np.float64)
+        CB = np.asfortranarray(CB, dtype=np.float64)
+        nv = self.chol_dims()["nvecs"]
+        NA, NB = a1 - a0 + 1, b1 - b0 + 1
+        out = np.zeros(nv * NA * NB)
+        _check(self.L.gi_mo_transform(self.h, _p(CA), CA.shape[0], a0, a1, _p(CB), CB.shape[0], b0, b1,
+                                      max_buffer_mb, flags, _p(out)))
+        return out.reshape(NA * NB, nv).T
+
+    def mo_transform_device(self, CA_ptr, lda, a0, a1, CB_ptr, ldb, b0, b1, Rkab_ptr, max_buffer_mb=1 << 20,
+                            flags=0):
+        _check(self.L.gi_mo_transform_device(self.h, C.c_void_p(CA_ptr), lda, a0, a1, C.c_void_p(CB_ptr), ldb,
+                                             b0, b1, max_buffer_mb, flags, C.c_void_p(Rkab_ptr)))
+
+
+def accuracy_tau(accuracy: int) -> float:
+    L = load()
+    t = C.c_double()
+    _check(L.gi_chol_accuracy_tau(accuracy, C.byref(t)))
+    return t.value

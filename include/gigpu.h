@@ -1,0 +1,136 @@
+/*
+ * gigpu -- MI355X (gfx950) engine for the gammcor-integrals hot path:
+ * two-step Cholesky decomposition of the Coulomb matrix (pq|rs) and the
+ * AO -> MO transform of its vectors.
+ *
+ * C ABI only: plain pointers, int64_t sizes, column-major arrays, int status
+ * (0 = success; otherwise gi_last_error() describes the failure).  Index
+ * arrays carry the 1-based values of the reference's Fortran arrays, so a
+ * Fortran caller passes TAOBasis / TChol2Vecs components unchanged through
+ * ISO_C_BINDING.  All calls are synchronous with respect to the host on
+ * return.  One context per (thread of control, device); a context is not
+ * re-entrant, like the reference's module state (Auto2eERI pointer table,
+ * Boys INTERPTABLE).
+ */
+#ifndef GIGPU_H
+#define GIGPU_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct gi_context gi_context;
+
+/* Accuracy levels of chol_gammcor_Rkpq -> tau
+ * (Cholesky_Gammcor.f90:14-17,398-410). */
+#define GI_CHOL_ACCURACY_DEFAULT 1   /* tau = 1e-5  */
+#define GI_CHOL_ACCURACY_TIGHT 2     /* tau = 1e-6  */
+#define GI_CHOL_ACCURACY_LUDICROUS 3 /* tau = 1e-7  */
+#define GI_CHOL_ACCURACY_DEBUG 4     /* tau = 1e-10 */
+
+/* gi_chol_build flags */
+#define GI_CHOL_DECODE_FIXED 1  /* Step 2 decodes same-shell pivots in the Step-1
+                                   storage order instead of chol2_pq2p_ge_q's
+                                   (TwoStepCholesky_Step2.f90:520-530) */
+#define GI_CHOL_SKIP_VECTORS 2  /* stop after Step 2 (no Rkpq) */
+#define GI_CHOL_TIME_ERI 4      /* time ERI launches with HIP events */
+#define GI_CHOL_RAW_TIES 8      /* compare the sort keys and the batch argmax on
+                                   raw doubles, as the reference does, instead of
+                                   keys quantized to 2^-40 (relative for the BASE
+                                   sort, of max(Vdiag) for residual diagonals) */
+
+/* gi_mo_transform flags */
+#define GI_MO_ALL_COLUMNS 1     /* transform every a column even when a0 > 1
+                                   (the reference's pass bounds,
+                                   Cholesky_Gammcor.f90:196-197, skip some) */
+
+/* Create a context on HIP device `device` (no reference counterpart: the
+ * reference initialises module state in chol_gammcor_Rkpq,
+ * Cholesky_Gammcor.f90:347-352). */
+int gi_create(gi_context **ctx, int64_t device);
+int gi_destroy(gi_context *ctx);
+/* Message of the last failure on this thread (replaces msg(...,MSG_ERROR) +
+ * error stop of the reference). */
+const char *gi_last_error(void);
+
+/* Upload the basis: the TAOBasis components (basis_definitions.f90:94-192)
+ * the reference passes to chol2_Step1_FullInterface
+ * (TwoStepCholesky.f90:74-105).  coords (3,natoms) bohr; shell_centers,
+ * shell_params (nshells) 1-based; momentum, nprim (nparams); cntr, exps
+ * (max_nprim, nparams) with primitive-normalized coefficients; norm
+ * (max_nang, nparams) = NormFactorsSpher if spher else NormFactorsCart. */
+int gi_set_basis(gi_context *ctx, int64_t natoms, const double *coords, int64_t nshells,
+                 const int64_t *shell_centers, const int64_t *shell_params, int64_t nparams,
+                 const int64_t *momentum, const int64_t *nprim, int64_t max_nprim,
+                 const double *cntr, const double *exps, int64_t max_nang, const double *norm,
+                 int64_t spher);
+
+/* Shell-quartet ERI blocks, the operation behind the reference's
+ * Auto2eERI(idx)%ptr(...) frontends (auto2e.f90:271-2880).  quartets
+ * (4, nq) holds 0-based shells (A,B,C,D); block q is written at
+ * out + offsets[q] as V(a,b,c,d) = (ab|cd), a fastest.  Host pointers. */
+int gi_eri_quartets(gi_context *ctx, int64_t nq, const int64_t *quartets, double kappa,
+                    const int64_t *offsets, double *out, int64_t out_len);
+
+/* Two-step Cholesky decomposition of the Coulomb matrix, the work of
+ * chol_gammcor_Rkpq -> chol_gammcor_CoulombMatrix (Cholesky_Gammcor.f90:335-441):
+ * Step 1 (pivots), Step 2 (Inv_L) and the full-dimension vectors Rkpq, all
+ * kept on the device.  tau: CholeskyTauThresh; kappa = 1/Omega^2 or 0;
+ * max_block_dim = MaxBlockDim (4000), vec_block = CholVecsBlock (500)
+ * (TwoStepCholesky_definitions.f90:20-53). */
+int gi_chol_build(gi_context *ctx, double tau, double kappa, int64_t max_block_dim,
+                  int64_t vec_block, int64_t flags);
+/* tau of a CHOL_ACCURACY_* level; returns 0 and sets *tau, or fails with
+ * "Invalid accuracy setting" (Cholesky_Gammcor.f90:407-409). */
+int gi_chol_accuracy_tau(int64_t accuracy, double *tau);
+
+/* Sizes of the TChol2Vecs components (TwoStepCholesky_definitions.f90:56-74):
+ * dims[0] NVecs, [1] NOrbPairs, [2] NShellPairs, [3] NSubsets(1),
+ * [4] MaxSubsetDim, [5] NPivotShellPairs, [6] macro iterations,
+ * [7] shell quartets evaluated. */
+int gi_chol_dims(gi_context *ctx, int64_t dims[8]);
+/* Fill caller-allocated TChol2Vecs arrays (any pointer may be NULL):
+ * ShellPairs(2,NShellPairs), ShellPairLoc(3,NShellPairs), ShellPairDim,
+ * SubsetDim, SubsetBounds(2,NSubsets), Pivots(NOrbPairs),
+ * pivot_order(NVecs) (FULL-storage locations in pivot order),
+ * PivotShellPairs, PivotShellPairLoc, PivotShellPairDim (NPivotShellPairs),
+ * PivotOrbPairs(NVecs). */
+int gi_chol_meta(gi_context *ctx, int64_t *shell_pairs, int64_t *shell_pair_loc,
+                 int64_t *shell_pair_dim, int64_t *subset_dim, int64_t *subset_bounds,
+                 int64_t *pivots, int64_t *pivot_order, int64_t *pivot_shell_pairs,
+                 int64_t *pivot_shell_pair_loc, int64_t *pivot_shell_pair_dim,
+                 int64_t *pivot_orb_pairs);
+/* Rkpq(NVecs, MaxSubsetDim, NSubsets) (TCholeskyVecsOTF%Rkpq,
+ * Cholesky_Gammcor.f90:19-22) copied to host memory. */
+int gi_chol_rkpq(gi_context *ctx, double *dst);
+/* Device address of Rkpq (valid until the next gi_chol_build / gi_destroy). */
+int gi_chol_rkpq_device(gi_context *ctx, double **ptr);
+/* Inv_L(NVecs, NVecs) and Vdiag(NAO(NAO+1)/2) to host memory. */
+int gi_chol_inv_l(gi_context *ctx, double *dst);
+int gi_chol_vdiag(gi_context *ctx, double *dst);
+/* Phase timings in seconds: [0] step 1, [1] step 2, [2] full vectors,
+ * [3] total build, [4] ERI kernel seconds (GI_CHOL_TIME_ERI), [5] ERI FP64
+ * operations, [6] Vdiag, [7] step-1 ERIs, [8] M - R^T R, [9] micro
+ * iterations, [10] V(P,P) ERIs, [11] POTRF+TRTRI, [12] Vabrs ERIs,
+ * [13] Inv_L V^T GEMM. */
+int gi_chol_timings(gi_context *ctx, double t[14]);
+
+/* AO -> MO transform of the Cholesky vectors, chol_gammcor_MOTransf_2
+ * (Cholesky_Gammcor.f90:143-332): Rkab(NVecs, NA*NB), a fastest,
+ * Rkab(k, a, b) = sum_pq R_k(p,q) CA(p,a) CB(q,b), a = a0..a1, b = b0..b1
+ * (1-based columns of CA (nao, *) ld lda and CB ld ldb, rows already in the
+ * Auto2e AO order as chol_gammcor_Rkab passes them, :74-83).  Fails with
+ * "Buffer size too small for Rpq->Rab" like :176-180.  Host pointers. */
+int gi_mo_transform(gi_context *ctx, const double *CA, int64_t lda, int64_t a0, int64_t a1,
+                    const double *CB, int64_t ldb, int64_t b0, int64_t b1,
+                    int64_t max_buffer_mb, int64_t flags, double *Rkab);
+/* Same with device pointers for CA, CB and Rkab. */
+int gi_mo_transform_device(gi_context *ctx, const double *CA, int64_t lda, int64_t a0,
+                           int64_t a1, const double *CB, int64_t ldb, int64_t b0, int64_t b1,
+                           int64_t max_buffer_mb, int64_t flags, double *Rkab);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
