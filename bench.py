@@ -85,6 +85,8 @@ def main():
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mo", action="store_true")
+    ap.add_argument("--eri-stats", action="store_true",
+                    help="print per-class ERI kernel statistics to stderr (sets GIGPU_ERI_STATS)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (see DESIGN.md, sharding)",
@@ -93,6 +95,8 @@ def main():
     if int(os.environ.get("RANK", "0")) != 0:
         return
 
+    if args.eri_stats:
+        os.environ["GIGPU_ERI_STATS"] = "1"
     from gigpu import lib, systems
     basis = systems.water_system(args.nwater, 10.0, "large", seed=args.seed)
     nocc, nvirt = 40, 300
@@ -168,6 +172,13 @@ def main():
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.seed)
+    if args.eri_stats:
+        rows = sorted(ctx.eri_class_stats(), key=lambda r: -r[5])
+        tot = sum(r[5] for r in rows)
+        print("class      quartets     ms    TFLOP/s  share", file=sys.stderr)
+        for la, lb, lc, ld, nq, ms, fl in rows:
+            print("(%d%d|%d%d) %11d %8.2f %8.3f %6.1f%%" % (la, lb, lc, ld, nq, ms, fl / ms / 1e9 if ms else 0,
+                                                         100 * ms / tot), file=sys.stderr)
     print(json.dumps(out))
     ctx.close()
 

@@ -286,6 +286,27 @@ int gi_chol_timings(gi_context *cp, double t[14]) {
   })
 }
 
+int gi_eri_class_stats(gi_context *cp, int64_t cap, double *rows, int64_t *n) {
+  GI_TRY({
+    gi::Context &C = ctx_of(cp);
+    int64_t k = 0;
+    for (auto &kv : C.class_stats) {
+      if (k >= cap) break;
+      int key = kv.first;
+      double *r = rows + 7 * k;
+      r[0] = (key / 64) / 8;
+      r[1] = (key / 64) % 8;
+      r[2] = (key % 64) / 8;
+      r[3] = key % 8;
+      r[4] = kv.second[0];
+      r[5] = kv.second[1];
+      r[6] = kv.second[2];
+      ++k;
+    }
+    *n = k;
+  })
+}
+
 int gi_mo_transform_device(gi_context *cp, const double *CA, int64_t lda, int64_t a0, int64_t a1,
                            const double *CB, int64_t ldb, int64_t b0, int64_t b1, int64_t max_buffer_mb,
                            int64_t flags, double *Rkab) {

@@ -76,63 +76,171 @@ static double qrel(double x) {
 }
 __host__ __device__ static inline double qabs(double x, double inv_delta) { return rint(x * inv_delta); }
 
-// first maximum (strict >, scan order) over D[bd[e]], e = 0..nb-1, compared on
-// quantized keys unless inv_delta == 0.  res[0] = raw value, res[1] = (double)e.
-// One workgroup of 1024 threads.
-__global__ void __launch_bounds__(1024) k_batchmax(const double *__restrict__ D, const long long *__restrict__ bd,
-                                                    long long nb, double inv_delta, double *__restrict__ res) {
-  __shared__ double sv[1024];
-  __shared__ long long si[1024];
-  double best = inv_delta > 0.0 ? qabs(-1.0, inv_delta) : -1.0;
-  long long bi = -1;
-  for (long long e = threadIdx.x; e < nb; e += blockDim.x) {
-    double v = D[bd[e]];
-    double k = inv_delta > 0.0 ? qabs(v, inv_delta) : v;
-    if (k > best) { best = k; bi = e; }   // e increases within a thread
-  }
-  sv[threadIdx.x] = best;
-  si[threadIdx.x] = bi;
+// ---- micro iterations restricted to the batch rows -------------------------
+// The pivot of every micro iteration is chosen among the batch elements, and
+// its row of L is a batch row, so the sequential part of chol2_Pivots only
+// needs L and D on the batch rows.  The loop runs on the device, one fused
+// kernel per micro iteration: column update of the batch rows, diagonal
+// update, and the first-maximum search finished by the last workgroup to
+// arrive (no host round trip per iteration).  The remaining candidate rows
+// are filled afterwards in one triangular solve (see chol_build).
+struct MicroState {
+  int done;       // loop has stopped
+  int j;          // columns computed in this macro iteration
+  int e;          // batch index of the current pivot
+  unsigned counter;
+  double val;     // raw residual diagonal of the current pivot
+};
+
+struct MicroPartial {
+  double key, val;
+  long long idx;
+};
+
+__device__ __forceinline__ void argmax_merge(double &k1, double &v1, long long &i1, double k2, double v2,
+                                             long long i2) {
+  bool take = (i2 >= 0) && (i1 < 0 || k2 > k1 || (k2 == k1 && i2 < i1));
+  if (take) { k1 = k2; v1 = v2; i1 = i2; }
+}
+
+template <int BS>
+__device__ void block_argmax(double &k, double &v, long long &i) {
+  __shared__ double sk[BS], sv[BS];
+  __shared__ long long si[BS];
+  sk[threadIdx.x] = k;
+  sv[threadIdx.x] = v;
+  si[threadIdx.x] = i;
   __syncthreads();
-  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+  for (int s = BS / 2; s > 0; s >>= 1) {
     if (threadIdx.x < s) {
-      double v2 = sv[threadIdx.x + s];
-      long long i2 = si[threadIdx.x + s];
-      double v1 = sv[threadIdx.x];
-      long long i1 = si[threadIdx.x];
-      // keep the larger; on ties the smaller index (first in scan order)
-      bool take = (i2 >= 0) && (i1 < 0 || v2 > v1 || (v2 == v1 && i2 < i1));
-      if (take) { sv[threadIdx.x] = v2; si[threadIdx.x] = i2; }
+      double kk = sk[threadIdx.x], vv = sv[threadIdx.x];
+      long long ii = si[threadIdx.x];
+      argmax_merge(kk, vv, ii, sk[threadIdx.x + s], sv[threadIdx.x + s], si[threadIdx.x + s]);
+      sk[threadIdx.x] = kk; sv[threadIdx.x] = vv; si[threadIdx.x] = ii;
     }
     __syncthreads();
   }
+  k = sk[0]; v = sv[0]; i = si[0];
+}
+
+// gather the batch block Mb(e, c) = M(bd(e), c), the batch diagonals, and the
+// initial first maximum
+__global__ void __launch_bounds__(1024) k_micro_init(const double *__restrict__ D, const long long *__restrict__ bd,
+                                                      long long nb, double inv_delta, double *__restrict__ Db,
+                                                      MicroState *st, int nsteps) {
+  double bk = inv_delta > 0.0 ? qabs(-1.0, inv_delta) : -1.0, bv = -1.0;
+  long long bi = -1;
+  for (long long e = threadIdx.x; e < nb; e += blockDim.x) {
+    double v = D[bd[e]];
+    Db[e] = v;
+    double k = inv_delta > 0.0 ? qabs(v, inv_delta) : v;
+    argmax_merge(bk, bv, bi, k, v, e);
+  }
+  block_argmax<1024>(bk, bv, bi);
   if (threadIdx.x == 0) {
-    const long long e = si[0];
-    res[0] = e >= 0 ? D[bd[e]] : -1.0;
-    res[1] = (double)e;
+    st->j = 0;
+    st->e = (int)bi;
+    st->val = bv;
+    st->counter = 0;
+    st->done = (bi < 0 || nsteps <= 0) ? 1 : 0;
   }
 }
 
-// one micro iteration (chol2_Lj + D update, TwoStepCholesky_Step1.f90:859-881,1044-1046):
-//   L(:, j) = (M(:, col) - L(:, j0:j-1) L(row, j0:j-1)^T) / sqrt(dmax)
-//   D -= L(:, j)^2
-// L is (ncand x cap) column-major, vectors are columns; col/row/dmax come
-// from the batch argmax result on the device.
-__global__ void __launch_bounds__(256) k_lj(const double *__restrict__ M, long long ldM,
-                                            double *__restrict__ L, long long ncand, long long j0, long long j,
-                                            const long long *__restrict__ bd, const double *__restrict__ bm,
-                                            double *__restrict__ D) {
+__global__ void k_gather_block(double *__restrict__ Mb, const double *__restrict__ M, long long ldM,
+                               const long long *__restrict__ bd, long long nb) {
+  long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  long long c = blockIdx.y;
+  if (e < nb && c < nb) Mb[e + nb * c] = M[bd[e] + ldM * c];
+}
+
+template <int BS>
+__global__ void __launch_bounds__(BS) k_micro_step(MicroState *st, const double *__restrict__ Mb,
+                                                   double *__restrict__ Lb, double *__restrict__ Db, long long nb,
+                                                   int nsteps, double dmin, double inv_delta,
+                                                   MicroPartial *__restrict__ part, int *__restrict__ piv) {
+  if (st->done) return;   // uniform: written by the previous launch
+  const int j = st->j, e = st->e;
+  const double val = st->val;
+  const double sq = sqrt(val);
+  const long long i = blockIdx.x * (long long)BS + threadIdx.x;
+  double bk = inv_delta > 0.0 ? qabs(-1.0, inv_delta) : -1.0, bv = -1.0;
+  long long bi = -1;
+  if (i < nb) {
+    double s = Mb[i + nb * e];
+    for (int q = 0; q < j; ++q) s -= Lb[i + nb * q] * Lb[e + nb * q];
+    const double l = s / sq;
+    Lb[i + nb * j] = l;
+    const double d = Db[i] - l * l;
+    Db[i] = d;
+    bk = inv_delta > 0.0 ? qabs(d, inv_delta) : d;
+    bv = d;
+    bi = i;
+  }
+  block_argmax<BS>(bk, bv, bi);
+  __shared__ unsigned ticket;
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = MicroPartial{bk, bv, bi};
+    __threadfence();
+    ticket = atomicAdd(&st->counter, 1u);
+  }
+  __syncthreads();
+  if (ticket != gridDim.x - 1) return;
+  // last workgroup of this step: finish the search, advance the state
+  __threadfence();
+  double k = inv_delta > 0.0 ? qabs(-1.0, inv_delta) : -1.0, v = -1.0;
+  long long ix = -1;
+  for (unsigned b = threadIdx.x; b < gridDim.x; b += BS) {
+    MicroPartial p = part[b];
+    argmax_merge(k, v, ix, p.key, p.val, p.idx);
+  }
+  block_argmax<BS>(k, v, ix);
+  if (threadIdx.x == 0) {
+    piv[j] = e;
+    const int jn = j + 1;
+    st->j = jn;
+    st->e = (int)ix;
+    st->val = v;
+    st->counter = 0;
+    st->done = (ix < 0 || v <= dmin || jn >= nb || jn >= nsteps) ? 1 : 0;
+  }
+}
+
+// Lt(:, j0+q) = M(:, piv(q))  (right-hand sides of the triangular solve)
+__global__ void k_gather_cols(double *__restrict__ Lt, long long ldL, const double *__restrict__ M, long long ldM,
+                              const int *__restrict__ piv, long long nrows) {
+  long long r = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  long long q = blockIdx.y;
+  if (r < nrows) Lt[r + ldL * q] = M[r + ldM * piv[q]];
+}
+
+// L_P(q, r) = Lb(piv(q), r) for r <= q, zero above the diagonal
+__global__ void k_build_lp(double *__restrict__ LP, long long J, const double *__restrict__ Lb, long long nb,
+                           const int *__restrict__ piv) {
+  long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  long long r = blockIdx.y;
+  if (q < J && r < J) LP[q + J * r] = (r <= q) ? Lb[piv[q] + nb * r] : 0.0;
+}
+
+// batch rows keep the values of the sequential recurrence; then
+// D(i) -= L(i, j0+q)^2 for q = 0..J-1 in order
+__global__ void k_finish_rows(double *__restrict__ Lt, long long ldL, long long j0, long long J,
+                              const double *__restrict__ Lb, long long nb, const long long *__restrict__ bd,
+                              double *__restrict__ D, long long nrows) {
+  long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (e < nb)
+    for (long long q = 0; q < J; ++q) Lt[bd[e] + ldL * (j0 + q)] = Lb[e + nb * q];
+}
+
+__global__ void k_update_d(const double *__restrict__ Lt, long long ldL, long long j0, long long J,
+                           double *__restrict__ D, long long nrows) {
   long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  if (i >= ncand) return;
-  const long long e = (long long)bm[1];
-  const long long row = bd[e];
-  const double sq = sqrt(bm[0]);
-  double s = M[i + ldM * e];
-  const double *Li = L + i;
-  const double *Lr = L + row;
-  for (long long q = j0; q < j; ++q) s -= Li[ncand * q] * Lr[ncand * q];
-  const double l = s / sq;
-  L[i + ncand * j] = l;
-  D[i] = D[i] - l * l;
+  if (i >= nrows) return;
+  double d = D[i];
+  for (long long q = 0; q < J; ++q) {
+    const double l = Lt[i + ldL * (j0 + q)];
+    d = d - l * l;
+  }
+  D[i] = d;
 }
 
 __global__ void k_blockmax(const double *__restrict__ D, long long n, double *__restrict__ part) {
@@ -362,7 +470,10 @@ void chol_build(Context &C, double tau, double kappa, long long max_block_dim, l
     long long nsps_cand = nsp, nops_cand = nop;
     std::vector<long long> cand(nsp + 1);   // candidate shell pairs, compressed order
     for (long long k = 1; k <= nsp; ++k) { cand[k] = k; loc_comp[k] = loc_full[k]; }
-    DBuf<double> dD, dDn, dL, dLn, dM, dLB, dtmp, dbm;
+    DBuf<double> dD, dDn, dL, dLn, dM, dLB, dtmp, dbm, dMb, dLb, dDb, dLP;
+    DBuf<int> dpiv;
+    DBuf<MicroState> dstate;
+    DBuf<MicroPartial> dpart;
     dD.upload(Dh.data(), nop, C.stream);
     long long ncand = nop;       // rows of L / length of D
     long long capL = 0;          // columns allocated in L
@@ -511,32 +622,60 @@ void chol_build(Context &C, double tau, double kappa, long long max_block_dim, l
         GI_HIP(hipStreamSynchronize(C.stream));
         t_rtr += now() - tr;
       }
-      // micro iterations
+      // micro iterations (chol2_Lj + D update + BatchMax, TwoStepCholesky_Step1.f90:1027-1064):
+      // the sequential recurrence runs on the batch rows on the device; the
+      // other candidate rows follow from one triangular solve
+      //   L(:, j0:j0+J) = M(:, P) L_P^-T,  L_P(q, r) = L(p_q, r),
+      // which is the same forward substitution row by row.
       double tm = now();
-      double bmh[2] = {-1.0, -1.0};
-      auto batchmax = [&]() {
-        if (nops_batch == 0) { bmh[0] = -1.0; bmh[1] = -1.0; return; }
-        hipLaunchKernelGGL(k_batchmax, dim3(1), dim3(1024), 0, C.stream, dD.p, dbd.p, nops_batch,
-                           raw_ties ? 0.0 : inv_delta, dbm.p);
-        GI_HIP(hipMemcpyAsync(bmh, dbm.p, sizeof(double) * 2, hipMemcpyDeviceToHost, C.stream));
-        GI_HIP(hipStreamSynchronize(C.stream));
-      };
-      batchmax();
       const long long j0 = nchol;
-      long long j = 0;
-      bool do_micro = nchol < max_nchol && nops_batch > 0;
-      while (do_micro) {
-        if (bmh[1] < 0.0) break;   // no batch element above -1: nothing to pivot on
-        ++j;
-        ++nchol;
-        const long long e = (long long)bmh[1];
-        pivots[bfull[e]] = 1;
-        pivot_order.push_back(bfull[e]);
-        hipLaunchKernelGGL(k_lj, dim3((unsigned)((ncand + 255) / 256)), dim3(256), 0, C.stream, dM.p, ncand, dL.p,
-                           ncand, j0, nchol - 1, dbd.p, dbm.p, dD.p);
-        batchmax();
-        if (bmh[0] <= dmin) do_micro = false;
-        if (j == nops_batch || nchol == max_nchol) do_micro = false;
+      const long long nb = nops_batch;
+      const int nsteps = (int)std::min<long long>(nb, max_nchol - nchol);
+      long long J = 0;
+      if (nchol < max_nchol && nb > 0) {
+        dMb.alloc((size_t)nb * nb);
+        dLb.alloc((size_t)nb * std::max(nsteps, 1));
+        dDb.alloc(nb);
+        dpiv.alloc(nb);
+        dstate.alloc(1);
+        const int BS = 256;
+        const int nblk = (int)((nb + BS - 1) / BS);
+        dpart.alloc(nblk);
+        hipLaunchKernelGGL(k_gather_block, dim3((unsigned)((nb + 255) / 256), (unsigned)nb), dim3(256), 0, C.stream,
+                           dMb.p, dM.p, ncand, dbd.p, nb);
+        const double qd = raw_ties ? 0.0 : inv_delta;
+        hipLaunchKernelGGL(k_micro_init, dim3(1), dim3(1024), 0, C.stream, dD.p, dbd.p, nb, qd, dDb.p, dstate.p,
+                           nsteps);
+        for (int s = 0; s < nsteps; ++s)
+          hipLaunchKernelGGL(k_micro_step<256>, dim3(nblk), dim3(BS), 0, C.stream, dstate.p, dMb.p, dLb.p, dDb.p,
+                             nb, nsteps, dmin, qd, dpart.p, dpiv.p);
+        MicroState hs{};
+        GI_HIP(hipMemcpyAsync(&hs, dstate.p, sizeof(hs), hipMemcpyDeviceToHost, C.stream));
+        GI_HIP(hipStreamSynchronize(C.stream));
+        J = hs.j;
+        std::vector<int> hpiv(std::max<long long>(J, 1));
+        if (J > 0) {
+          GI_HIP(hipMemcpy(hpiv.data(), dpiv.p, sizeof(int) * J, hipMemcpyDeviceToHost));
+          for (long long q = 0; q < J; ++q) {
+            pivots[bfull[hpiv[q]]] = 1;
+            pivot_order.push_back(bfull[hpiv[q]]);
+          }
+          // all candidate rows: right-hand sides, L_P, triangular solve
+          hipLaunchKernelGGL(k_gather_cols, dim3((unsigned)((ncand + 255) / 256), (unsigned)J), dim3(256), 0,
+                             C.stream, dL.p + (size_t)ncand * j0, ncand, dM.p, ncand, dpiv.p, ncand);
+          dLP.alloc((size_t)J * J);
+          hipLaunchKernelGGL(k_build_lp, dim3((unsigned)((J + 255) / 256), (unsigned)J), dim3(256), 0, C.stream,
+                             dLP.p, J, dLb.p, nb, dpiv.p);
+          const double one = 1.0;
+          ROCBLAS_CHECK(rocblas_dtrsm(C.blas, rocblas_side_right, rocblas_fill_lower, rocblas_operation_transpose,
+                                      rocblas_diagonal_non_unit, ncand, J, &one, dLP.p, J,
+                                      dL.p + (size_t)ncand * j0, ncand));
+          hipLaunchKernelGGL(k_finish_rows, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, C.stream, dL.p, ncand,
+                             j0, J, dLb.p, nb, dbd.p, dD.p, ncand);
+          hipLaunchKernelGGL(k_update_d, dim3((unsigned)((ncand + 255) / 256)), dim3(256), 0, C.stream, dL.p,
+                             ncand, j0, J, dD.p, ncand);
+        }
+        nchol += J;
       }
       t_micro += now() - tm;
       dmax = (ncand > 0) ? device_max(C, dD.p, ncand, dtmp) : 0.0;

@@ -143,6 +143,7 @@ void eri_run(Context &C, const EriRequest &R) {
     GI_HIP(hipEventRecord(e0, C.stream));
   }
   long long nq_total = 0, flops_total = 0;
+  static const bool stats = getenv("GIGPU_ERI_STATS") != nullptr;
   for (size_t i = 0; i < launches.size(); ++i) {
     auto &L = launches[i];
     int l[4] = {std::get<0>(L.rk), std::get<1>(L.rk), std::get<0>(L.ck), std::get<1>(L.ck)};
@@ -177,8 +178,26 @@ void eri_run(Context &C, const EriRequest &R) {
     long long ntask = (R.mode == LAUNCH_PRODUCT) ? a.nrows * a.ncols : a.nrows;
     if (ntask == 0) continue;
     long long ngroups = (ntask + ent.plan.qpw - 1) / ent.plan.qpw;
+    hipEvent_t s0 = nullptr, s1 = nullptr;
+    if (stats) {
+      GI_HIP(hipEventCreate(&s0));
+      GI_HIP(hipEventCreate(&s1));
+      GI_HIP(hipEventRecord(s0, C.stream));
+    }
     ent.fn(a, ngroups, C.stream);
     GI_HIP(hipGetLastError());
+    if (stats) {
+      GI_HIP(hipEventRecord(s1, C.stream));
+      GI_HIP(hipEventSynchronize(s1));
+      float ms = 0;
+      GI_HIP(hipEventElapsedTime(&ms, s0, s1));
+      auto &st = C.class_stats[(lc[0] * 8 + lc[1]) * 64 + lc[2] * 8 + lc[3]];
+      st[0] += ntask;
+      st[1] += ms;
+      st[2] += (double)ntask * ent.plan.flops * (np[0] * np[1] * np[2] * np[3]);
+      (void)hipEventDestroy(s0);
+      (void)hipEventDestroy(s1);
+    }
     nq_total += ntask;
     flops_total += ntask * ent.plan.flops * (long long)(np[0] * np[1] * np[2] * np[3]);
   }

@@ -2,6 +2,7 @@
 // basis, the Boys table, the ERI workspace and the Cholesky results.
 #pragma once
 #include <rocblas/rocblas.h>
+#include <array>
 #include <map>
 #include <memory>
 #include <vector>
@@ -83,6 +84,9 @@ struct Context {
   double eri_ms_acc = 0;
   long long eri_quartets_acc = 0;
   long long eri_flops_acc = 0;
+  // per canonical class (key (la*8+lb)*64+lc*8+ld): quartets, kernel ms, FP64 ops
+  // (filled when GIGPU_ERI_STATS is set in the environment)
+  std::map<int, std::array<double, 3>> class_stats;
 
   Context(int dev);
   ~Context();

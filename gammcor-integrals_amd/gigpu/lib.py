@@ -16,7 +16,7 @@ LIB_PATH = os.path.join(HERE, "libgigpu.so")
 EXPORTS = ["gi_create", "gi_destroy", "gi_last_error", "gi_set_basis", "gi_eri_quartets",
            "gi_chol_build", "gi_chol_accuracy_tau", "gi_chol_dims", "gi_chol_meta", "gi_chol_rkpq",
            "gi_chol_rkpq_device", "gi_chol_inv_l", "gi_chol_vdiag", "gi_chol_timings",
-           "gi_mo_transform", "gi_mo_transform_device"]
+           "gi_mo_transform", "gi_mo_transform_device", "gi_eri_class_stats"]
 
 CHOL_DECODE_FIXED = 1
 CHOL_SKIP_VECTORS = 2
@@ -57,6 +57,7 @@ def load():
     L.gi_chol_inv_l.argtypes = [C.c_void_p, dptr]
     L.gi_chol_vdiag.argtypes = [C.c_void_p, dptr]
     L.gi_chol_timings.argtypes = [C.c_void_p, dptr]
+    L.gi_eri_class_stats.argtypes = [C.c_void_p, i64, dptr, iptr]
     L.gi_mo_transform.argtypes = [C.c_void_p, dptr, i64, i64, i64, dptr, i64, i64, i64, i64, i64, dptr]
     L.gi_mo_transform_device.argtypes = [C.c_void_p, C.c_void_p, i64, i64, i64, C.c_void_p, i64, i64, i64, i64,
                                          i64, C.c_void_p]
@@ -187,6 +188,13 @@ class Context:
         keys = ["step1", "step2", "vectors", "total", "eri_kernel", "eri_flops", "vdiag", "eri_step1", "rtr",
                 "micro", "vpqrs", "potrf_trtri", "vabrs", "vectors_gemm"]
         return dict(zip(keys, t.tolist()))
+
+    def eri_class_stats(self):
+        """[(la, lb, lc, ld, quartets, kernel_ms, fp64_ops)] (needs GIGPU_ERI_STATS)."""
+        rows = np.zeros(7 * 512)
+        n = C.c_int64()
+        _check(self.L.gi_eri_class_stats(self.h, 512, _p(rows), C.byref(n)))
+        return [tuple(rows[7 * k:7 * k + 7]) for k in range(n.value)]
 
     def mo_transform(self, CA, a0, a1, CB, b0, b1, max_buffer_mb=1 << 20, flags=0):
         CA = np.asfortranarray(CA, dtype=np.float64)
