@@ -173,12 +173,12 @@ def main():
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.seed)
     if args.eri_stats:
-        rows = sorted(ctx.eri_class_stats(), key=lambda r: -r[5])
-        tot = sum(r[5] for r in rows)
-        print("class      quartets     ms    TFLOP/s  share", file=sys.stderr)
-        for la, lb, lc, ld, nq, ms, fl in rows:
-            print("(%d%d|%d%d) %11d %8.2f %8.3f %6.1f%%" % (la, lb, lc, ld, nq, ms, fl / ms / 1e9 if ms else 0,
-                                                         100 * ms / tot), file=sys.stderr)
+        rows = sorted(ctx.eri_class_stats(), key=lambda r: -r[3])
+        tot = sum(r[3] for r in rows)
+        print("E F      quartets     ms    TFLOP/s  share", file=sys.stderr)
+        for E, F, nq, ms, fl in rows:
+            print("%d %d %13d %8.2f %8.3f %6.1f%%" % (E, F, nq, ms, fl / ms / 1e9 if ms else 0,
+                                                    100 * ms / tot), file=sys.stderr)
     print(json.dumps(out))
     ctx.close()
 

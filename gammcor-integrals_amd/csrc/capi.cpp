@@ -156,11 +156,8 @@ int gi_eri_quartets(gi_context *cp, int64_t nq, const int64_t *quartets, double 
         cd.push_back(gi::PairDesc{(int)quartets[4 * q + 2], (int)quartets[4 * q + 3], 0, 0, 0, 0, 0});
         rows.push_back((int)rows.size());
       }
-      gi::DBuf<gi::PairDesc> drd, dcd;
-      drd.upload(rd, C.stream);
-      dcd.upload(cd, C.stream);
       gi::EriRequest R;
-      R.rdesc = &rd; R.cdesc = &cd; R.d_rdesc = drd.p; R.d_cdesc = dcd.p;
+      R.rdesc = &rd; R.cdesc = &cd;
       R.rows = &rows; R.cols = &rows; R.mode = gi::LAUNCH_ZIP;
       R.out = dout.p; R.ldo = g.first; R.kappa = kappa;
       gi::eri_run(C, R);
@@ -293,14 +290,12 @@ int gi_eri_class_stats(gi_context *cp, int64_t cap, double *rows, int64_t *n) {
     for (auto &kv : C.class_stats) {
       if (k >= cap) break;
       int key = kv.first;
-      double *r = rows + 7 * k;
-      r[0] = (key / 64) / 8;
-      r[1] = (key / 64) % 8;
-      r[2] = (key % 64) / 8;
-      r[3] = key % 8;
-      r[4] = kv.second[0];
-      r[5] = kv.second[1];
-      r[6] = kv.second[2];
+      double *r = rows + 5 * k;
+      r[0] = key / 16;
+      r[1] = key % 16;
+      r[2] = kv.second[0];
+      r[3] = kv.second[1];
+      r[4] = kv.second[2];
       ++k;
     }
     *n = k;

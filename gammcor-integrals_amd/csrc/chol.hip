@@ -350,11 +350,8 @@ void chol_build(Context &C, double tau, double kappa, long long max_block_dim, l
       cd[k - 1] = PairDesc{sa, sb, fl, 0, 0, 0, 0};
       rows[k - 1] = (int)(k - 1);
     }
-    DBuf<PairDesc> drd, dcd;
-    drd.upload(rd, C.stream);
-    dcd.upload(cd, C.stream);
     EriRequest R;
-    R.rdesc = &rd; R.cdesc = &cd; R.d_rdesc = drd.p; R.d_cdesc = dcd.p;
+    R.rdesc = &rd; R.cdesc = &cd;
     R.rows = &rows; R.cols = &rows; R.mode = LAUNCH_DIAG;
     R.out = d_vdiag.p; R.ldo = 0; R.kappa = kappa;
     eri_run(C, R);
@@ -598,11 +595,8 @@ void chol_build(Context &C, double tau, double kappa, long long max_block_dim, l
           cols[k - 1] = (int)(k - 1);
         }
         if (nsps_batch > 0 && nsps_cand > 0) {
-          DBuf<PairDesc> drd, dcd;
-          drd.upload(rd, C.stream);
-          dcd.upload(cd, C.stream);
           EriRequest R;
-          R.rdesc = &rd; R.cdesc = &cd; R.d_rdesc = drd.p; R.d_cdesc = dcd.p;
+          R.rdesc = &rd; R.cdesc = &cd;
           R.rows = &rows; R.cols = &cols; R.mode = LAUNCH_PRODUCT;
           R.out = dM.p; R.ldo = ncand; R.kappa = kappa;
           eri_run(C, R);
@@ -782,11 +776,8 @@ void chol_build(Context &C, double tau, double kappa, long long max_block_dim, l
       cd[k] = PairDesc{spA[ab], spB[ab], PAIR_SEL, (int)(psploc[k] - 1), (int)pspdim[k], 0, nvec * (psploc[k] - 1)};
       rows[k] = (int)k;
     }
-    DBuf<PairDesc> drd, dcd;
-    drd.upload(rd, C.stream);
-    dcd.upload(cd, C.stream);
     EriRequest R;
-    R.rdesc = &rd; R.cdesc = &cd; R.d_rdesc = drd.p; R.d_cdesc = dcd.p;
+    R.rdesc = &rd; R.cdesc = &cd;
     R.rows = &rows; R.cols = &rows; R.mode = LAUNCH_PRODUCT;
     R.out = S.inv_l.p; R.ldo = nvec; R.d_sel = dsel.p; R.kappa = kappa;
     eri_run(C, R);
@@ -831,8 +822,6 @@ void chol_build(Context &C, double tau, double kappa, long long max_block_dim, l
       cd[k] = PairDesc{spA[ab], spB[ab], PAIR_SEL, (int)(psploc[k] - 1), (int)pspdim[k], 0, msd * (psploc[k] - 1)};
       cols[k] = (int)k;
     }
-    DBuf<PairDesc> dcd;
-    dcd.upload(cd, C.stream);
     double t_vab = 0, t_gemm = 0;
     for (long long X0 = 1; X0 <= nsub; X0 += per_group) {
       long long X1 = std::min(nsub, X0 + per_group - 1);
@@ -851,10 +840,8 @@ void chol_build(Context &C, double tau, double kappa, long long max_block_dim, l
       double tv = now();
       if (!rd.empty() && npsp > 0) {
         EriPhaseTimer pt(C, "vabrs");
-        DBuf<PairDesc> drd;
-        drd.upload(rd, C.stream);
         EriRequest R;
-        R.rdesc = &rd; R.cdesc = &cd; R.d_rdesc = drd.p; R.d_cdesc = dcd.p;
+        R.rdesc = &rd; R.cdesc = &cd;
         R.rows = &rows; R.cols = &cols; R.mode = LAUNCH_PRODUCT;
         R.out = dW.p; R.ldo = msd; R.d_sel = dsel.p; R.kappa = kappa;
         eri_run(C, R);

@@ -1,114 +1,272 @@
-// ERI launch layer: instantiates one kernel per canonical class (list emitted
-// by gigpu/eri_gen.py into eri_classes.inc), bins shell pairs by angular
-// momenta and primitive counts, and launches the Cartesian product (or the
-// zipped pairing) of every row bin with every column bin.
+// ERI launch layer of the Hermite-matrix engine: deduplicates the shell pairs
+// of a request into a pair table, builds their Hermite coefficient tables
+// (k_build_et), bins bra pairs by E = l0 + l1 and ket pairs by F = l2 + l3,
+// and launches md_kernel<E, F> on the Cartesian product (or the element-wise
+// pairing) of every bra bin with every ket bin.  One instantiation per
+// (E, F), listed by gigpu/eri_gen.py in md_classes.inc.
 #include <algorithm>
 #include <cstdio>
 #include <map>
-#include <tuple>
+#include <unordered_map>
 #include "gi_context.h"
 
 namespace gi {
 
 __constant__ double c_spher[SPHER_TOTAL];
+__constant__ int c_herm[NHRMAX];
+__constant__ int c_rrec[NHRMAX];
 
-using LaunchFn = void (*)(const EriArgs &, long long ngroups, hipStream_t);
-
-template <int LA, int LB, int LC, int LD>
-static void launch_class(const EriArgs &a, long long ngroups, hipStream_t s) {
-  using P = EriPlan<LA, LB, LC, LD>;
-  long long grid = ngroups;
-  if (P::MODE == MODE_GLB && grid > 2048) grid = 2048;
-  if (grid > 0x7fffffffLL) grid = 0x7fffffffLL;
-  hipLaunchKernelGGL((eri_class_kernel<LA, LB, LC, LD>), dim3((unsigned)grid), dim3(64), 0, s, a);
-}
-
-struct ClassEntry {
-  LaunchFn fn;
-  ClassPlan plan;
-};
-
-static std::map<int, ClassEntry> &class_table() {
-  static std::map<int, ClassEntry> t;
-  if (t.empty()) {
-#define GI_CLASS(a, b, c, d, fl)                                                               \
-  t[((a)*8 + (b)) * 64 + (c)*8 + (d)] = ClassEntry{                                           \
-      &launch_class<a, b, c, d>,                                                              \
-      ClassPlan{EriPlan<a, b, c, d>::MODE, EriPlan<a, b, c, d>::G, EriPlan<a, b, c, d>::QPW, \
-                EriPlan<a, b, c, d>::PER_Q, (long long)(fl)}};
-#include "eri_classes.inc"
-#undef GI_CLASS
+// ------------------------------------------------------------- pair tables
+// ET(h, ma + n0 mb) of one primitive pair: two-centre 1D Hermite coefficients
+//   E^{i+1,j}_t = E^{ij}_{t-1}/(2p) + X_PA E^{ij}_t + (t+1) E^{ij}_{t+1}
+//   E^{i,j+1}_t = E^{ij}_{t-1}/(2p) + X_PB E^{ij}_t + (t+1) E^{ij}_{t+1}
+// multiplied over x, y, z, transformed to real solid harmonics on both
+// shells (unit-sphere normalized, m = -l..l, auto2e_SpherAO.F90 convention),
+// times the AO normalization factors, the contraction coefficients and
+// exp(-ab/p |AB|^2).  One 64-lane workgroup per primitive pair.
+constexpr int ET_HCH = 8;   // h values per LDS chunk
+__global__ void __launch_bounds__(64) k_build_et(const DevShell *__restrict__ shells, const double *__restrict__ exps,
+                                                 const double *__restrict__ cntr, const double *__restrict__ norms,
+                                                 int spher, const EtJob *__restrict__ jobs, PrimRec *prims,
+                                                 double *et) {
+  constexpr int N1D = (MAXL + 1) * (MAXL + 1) * (2 * MAXL + 1);
+  __shared__ double s_e1[3][N1D];
+  __shared__ double s_t[ncart(MAXL) * ncart(MAXL) * ET_HCH];
+  const int lane = threadIdx.x;
+  const EtJob J = jobs[blockIdx.x];
+  const DevShell A = shells[J.s0], B = shells[J.s1];
+  const int la = A.l, lb = B.l, lab = la + lb;
+  const double a = exps[A.prim0 + J.ia], b = exps[B.prim0 + J.ib];
+  const double p = a + b;
+  const double P[3] = {(a * A.x + b * B.x) / p, (a * A.y + b * B.y) / p, (a * A.z + b * B.z) / p};
+  const double AB2 = (A.x - B.x) * (A.x - B.x) + (A.y - B.y) * (A.y - B.y) + (A.z - B.z) * (A.z - B.z);
+  const double pref = cntr[A.prim0 + J.ia] * cntr[B.prim0 + J.ib] * exp(-a * b / p * AB2);
+  if (lane == 0) prims[J.prim] = PrimRec{P[0], P[1], P[2], p, J.et, 0};
+  const int s_i = (lb + 1) * (lab + 1), s_j = lab + 1;   // E1(i, j, t) at i*s_i + j*s_j + t
+  if (lane < 3) {
+    const double Ac = lane == 0 ? A.x : (lane == 1 ? A.y : A.z);
+    const double Bc = lane == 0 ? B.x : (lane == 1 ? B.y : B.z);
+    const double xpa = P[lane] - Ac, xpb = P[lane] - Bc, i2p = 0.5 / p;
+    double *E1 = s_e1[lane];
+    for (int k = 0; k < (la + 1) * s_i; ++k) E1[k] = 0.0;
+    E1[0] = 1.0;
+    for (int i = 0; i < la; ++i)
+      for (int t = 0; t <= i + 1; ++t) {
+        double v = xpa * (t <= i ? E1[i * s_i + t] : 0.0);
+        if (t >= 1) v += i2p * E1[i * s_i + t - 1];
+        if (t + 1 <= i) v += (double)(t + 1) * E1[i * s_i + t + 1];
+        E1[(i + 1) * s_i + t] = v;
+      }
+    for (int i = 0; i <= la; ++i)
+      for (int j = 0; j < lb; ++j)
+        for (int t = 0; t <= i + j + 1; ++t) {
+          const int o = i * s_i + j * s_j;
+          double v = xpb * (t <= i + j ? E1[o + t] : 0.0);
+          if (t >= 1) v += i2p * E1[o + t - 1];
+          if (t + 1 <= i + j) v += (double)(t + 1) * E1[o + t + 1];
+          E1[o + s_j + t] = v;
+        }
   }
-  return t;
+  __syncthreads();
+  const int nc0 = ncart(la), nc1 = ncart(lb);
+  const bool t0 = spher && la >= 2, t1 = spher && lb >= 2;
+  const int n0 = t0 ? 2 * la + 1 : nc0, n1 = t1 ? 2 * lb + 1 : nc1;
+  const double *U0 = c_spher + spher_off(la >= 2 ? la : 2);
+  const double *U1 = c_spher + spher_off(lb >= 2 ? lb : 2);
+  const double *nA = norms + A.norm0, *nB = norms + B.norm0;
+  const int NH = nherm(lab);
+  const int nab = n0 * n1;
+  double *out = et + J.et;
+  for (int h0 = 0; h0 < NH; h0 += ET_HCH) {
+    // pass 1: T(a, mb, hh) = sum_b U1(mb, b) E(a, b, h0 + hh)
+    for (int e = lane; e < nc0 * n1 * ET_HCH; e += 64) {
+      const int hh = e % ET_HCH, rest = e / ET_HCH;
+      const int mb = rest % n1, ia = rest / n1;
+      const int h = h0 + hh;
+      double acc = 0.0;
+      if (h < NH) {
+        const int th = c_herm[h];
+        const int tt = px(th), uu = py(th), vv = pz(th);
+        const int ca = cart_of(la, ia);
+        const int ax = px(ca), ay = py(ca), az = pz(ca);
+        for (int ib = 0; ib < nc1; ++ib) {
+          const double u = t1 ? U1[mb * nc1 + ib] : (ib == mb ? 1.0 : 0.0);
+          if (u == 0.0) continue;
+          const int cb = cart_of(lb, ib);
+          const int bx = px(cb), by = py(cb), bz = pz(cb);
+          if (tt > ax + bx || uu > ay + by || vv > az + bz) continue;
+          acc += u * s_e1[0][ax * s_i + bx * s_j + tt] * s_e1[1][ay * s_i + by * s_j + uu] *
+                 s_e1[2][az * s_i + bz * s_j + vv];
+        }
+      }
+      s_t[e] = acc;
+    }
+    __syncthreads();
+    // pass 2: ET(h, ma + n0 mb) = pref nA(ma) nB(mb) sum_a U0(ma, a) T(a, mb, hh)
+    for (int e = lane; e < nab * ET_HCH; e += 64) {
+      const int hh = e / nab, r = e - hh * nab;
+      const int h = h0 + hh;
+      if (h >= NH) continue;
+      const int mb = r / n0, ma = r - mb * n0;
+      double acc = 0.0;
+      if (t0) {
+        for (int ia = 0; ia < nc0; ++ia) acc += U0[ma * nc0 + ia] * s_t[(ia * n1 + mb) * ET_HCH + hh];
+      } else {
+        acc = s_t[(ma * n1 + mb) * ET_HCH + hh];
+      }
+      out[(long long)h * nab + r] = acc * pref * nA[ma] * nB[mb];
+    }
+    __syncthreads();
+  }
 }
 
-const ClassPlan *class_plan(int la, int lb, int lc, int ld) {
-  auto &t = class_table();
-  auto it = t.find((la * 8 + lb) * 64 + lc * 8 + ld);
-  return it == t.end() ? nullptr : &it->second.plan;
+
+using LaunchFn = void (*)(const MdArgs &, int grid, hipStream_t);
+
+template <int E, int F>
+static void launch_md(const MdArgs &a, int grid, hipStream_t s) {
+  hipLaunchKernelGGL((md_kernel<E, F>), dim3((unsigned)grid), dim3(64), 0, s, a);
 }
 
-bool class_supported(int la, int lb, int lc, int ld) { return class_plan(la, lb, lc, ld) != nullptr; }
+static LaunchFn md_table[MD_EMAX + 1][MD_EMAX + 1];
+
+static void init_md_table() {
+  static bool done = false;
+  if (done) return;
+#define GI_MD(e, f) md_table[e][f] = &launch_md<e, f>;
+#include "md_classes.inc"
+#undef GI_MD
+  done = true;
+}
 
 void upload_constants(Context &C) {
+  init_md_table();
   std::vector<double> U(SPHER_TOTAL, 0.0);
   for (int l = 2; l <= MAXL; ++l) spher_u_host(l, U.data() + spher_off(l));
   GI_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_spher), U.data(), sizeof(double) * SPHER_TOTAL));
+  // Hermite index tables
+  std::vector<int> herm(NHRMAX), rrec(NHRMAX, 0);
+  for (int i = 0; i < NHRMAX; ++i) {
+    const int p = herm_of(i);
+    herm[i] = p;
+    const int t = px(p), u = py(p), v = pz(p);
+    if (i == 0) continue;
+    int dir, c;
+    if (t > 0) { dir = 0; c = t; }
+    else if (u > 0) { dir = 1; c = u; }
+    else { dir = 2; c = v; }
+    const int i1 = hidx(t - (dir == 0), u - (dir == 1), v - (dir == 2));
+    const int i2 = (c >= 2) ? hidx(t - 2 * (dir == 0), u - 2 * (dir == 1), v - 2 * (dir == 2)) : 0;
+    rrec[i] = i1 | (i2 << 11) | (dir << 22) | ((c - 1) << 24);
+  }
+  GI_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_herm), herm.data(), sizeof(int) * NHRMAX));
+  GI_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_rrec), rrec.data(), sizeof(int) * NHRMAX));
   std::vector<double> table;
   boys_table_host(BOYS_MAXM, table);
   C.dboys.upload(table, C.stream);
   GI_HIP(hipStreamSynchronize(C.stream));
 }
 
-// canonical order of (la, lb, lc, ld); m[k] = caller slot of canonical position k
-static void canonicalize(const int l_in[4], int lc_out[4], int m[4]) {
-  int l[4] = {l_in[0], l_in[1], l_in[2], l_in[3]};
-  for (int k = 0; k < 4; ++k) m[k] = k;
-  if (l[1] > l[0]) { std::swap(l[0], l[1]); std::swap(m[0], m[1]); }
-  if (l[3] > l[2]) { std::swap(l[2], l[3]); std::swap(m[2], m[3]); }
-  if (l[2] > l[0] || (l[2] == l[0] && l[3] > l[1])) {
-    std::swap(l[0], l[2]); std::swap(l[1], l[3]);
-    std::swap(m[0], m[2]); std::swap(m[1], m[3]);
-  }
-  for (int k = 0; k < 4; ++k) lc_out[k] = l[k];
+// FP64 operations of the R table of one primitive quartet of total order L:
+// Boys interpolation and recursion + 3 per recurrence entry
+static double rtable_flops(int L) {
+  double n = 0;
+  for (int m = 0; m <= L; ++m) n += nherm(m);
+  return 14.0 + 3.0 * L + 3.0 * n;
+}
+
+static int pair_rows_host(const PairDesc &d, int n0, int n1) {
+  if (d.flags & PAIR_SEL) return d.nsel;
+  if (d.flags & PAIR_SAME) return n0 * (n0 + 1) / 2;
+  return n0 * n1;
 }
 
 void eri_run(Context &C, const EriRequest &R) {
-  auto &tab = class_table();
-  // bin key of a pair: (l0, l1, np0, np1)
-  auto key_of = [&](const PairDesc &d) {
-    return std::make_tuple(C.shell_l[d.s0], C.shell_l[d.s1], C.shell_np[d.s0], C.shell_np[d.s1]);
+  init_md_table();
+  std::vector<PairDesc> rd = *R.rdesc, cd = *R.cdesc;
+  // ---- pair table
+  std::unordered_map<long long, int> pt_of;
+  std::vector<PairHdr> hdr;
+  std::vector<EtJob> jobs;
+  long long et_total = 0;
+  auto pair_entry = [&](PairDesc &d) {
+    const long long key = (long long)d.s0 * C.nshells + d.s1;
+    auto it = pt_of.find(key);
+    if (it != pt_of.end()) {
+      d.pt = it->second;
+      return;
+    }
+    const int l0 = C.shell_l[d.s0], l1 = C.shell_l[d.s1];
+    if (l0 + l1 > MD_EMAX) {
+      char buf[160];
+      snprintf(buf, sizeof buf, "no ERI kernel for a shell pair with l0 + l1 = %d (> %d)", l0 + l1, MD_EMAX);
+      throw Error{3, buf};
+    }
+    const int n0 = C.shell_nang[d.s0], n1 = C.shell_nang[d.s1];
+    const int np0 = C.shell_np[d.s0], np1 = C.shell_np[d.s1];
+    PairHdr h{n0, n1, np0 * np1, (int)jobs.size()};
+    for (int ib = 0; ib < np1; ++ib)
+      for (int ia = 0; ia < np0; ++ia) {
+        jobs.push_back(EtJob{d.s0, d.s1, ia, ib, (int)jobs.size(), 0, et_total});
+        et_total += (long long)nherm(l0 + l1) * n0 * n1;
+      }
+    d.pt = (int)hdr.size();
+    pt_of[key] = d.pt;
+    hdr.push_back(h);
   };
-  using Key = std::tuple<int, int, int, int>;
-  std::map<Key, std::vector<int>> rbins, cbins;
-  for (int r : *R.rows) rbins[key_of((*R.rdesc)[r])].push_back(r);
-  if (R.mode == LAUNCH_PRODUCT) {
-    for (int c : *R.cols) cbins[key_of((*R.cdesc)[c])].push_back(c);
+  for (int r : *R.rows) pair_entry(rd[r]);
+  for (int c : *R.cols) pair_entry(cd[c]);
+  for (int r : *R.rows) {
+    const PairDesc &d = rd[r];
+    if (pair_rows_host(d, C.shell_nang[d.s0], C.shell_nang[d.s1]) <= 0) throw Error{8, "empty bra pair"};
   }
-  // For zipped launches the row and column lists are paired element by
-  // element; bin by the combined key.
+  for (int c : *R.cols) {
+    const PairDesc &d = cd[c];
+    if (pair_rows_host(d, C.shell_nang[d.s0], C.shell_nang[d.s1]) <= 0) throw Error{8, "empty ket pair"};
+  }
+  if (jobs.empty()) return;
+  C.md_hdr.upload(hdr, C.stream);
+  C.md_jobs.upload(jobs, C.stream);
+  C.md_prim.alloc(jobs.size());
+  C.md_et.alloc((size_t)et_total);
+  C.md_rdesc.upload(rd, C.stream);
+  C.md_cdesc.upload(cd, C.stream);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (C.time_eri) {
+    GI_HIP(hipEventCreate(&e0));
+    GI_HIP(hipEventCreate(&e1));
+    GI_HIP(hipEventRecord(e0, C.stream));
+  }
+  hipLaunchKernelGGL(k_build_et, dim3((unsigned)jobs.size()), dim3(64), 0, C.stream, C.dshells.p, C.dexps.p,
+                     C.dcntr.p, C.dnorms.p, C.spher, C.md_jobs.p, C.md_prim.p, C.md_et.p);
+  GI_HIP(hipGetLastError());
+
+  // ---- bins by E (rows) and F (columns)
+  auto E_of = [&](const PairDesc &d) { return C.shell_l[d.s0] + C.shell_l[d.s1]; };
   struct Launch {
-    Key rk, ck;
+    int E, F;
     std::vector<int> rows, cols;
   };
   std::vector<Launch> launches;
   if (R.mode == LAUNCH_PRODUCT) {
-    for (auto &rb : rbins)
-      for (auto &cb : cbins) launches.push_back(Launch{rb.first, cb.first, rb.second, cb.second});
+    std::map<int, std::vector<int>> rb, cb;
+    for (int r : *R.rows) rb[E_of(rd[r])].push_back(r);
+    for (int c : *R.cols) cb[E_of(cd[c])].push_back(c);
+    for (auto &x : rb)
+      for (auto &y : cb) launches.push_back(Launch{x.first, y.first, x.second, y.second});
   } else {
-    std::map<std::pair<Key, Key>, Launch> zb;
+    std::map<std::pair<int, int>, Launch> zb;
     for (size_t i = 0; i < R.rows->size(); ++i) {
-      int r = (*R.rows)[i], c = (*R.cols)[i];
-      Key rk = key_of((*R.rdesc)[r]), ck = key_of((*R.cdesc)[c]);
-      auto &L = zb[{rk, ck}];
-      L.rk = rk;
-      L.ck = ck;
+      const int r = (*R.rows)[i], c = (*R.cols)[i];
+      const int E = E_of(rd[r]), F = E_of(cd[c]);
+      auto &L = zb[{E, F}];
+      L.E = E;
+      L.F = F;
       L.rows.push_back(r);
       L.cols.push_back(c);
     }
     for (auto &kv : zb) launches.push_back(std::move(kv.second));
   }
-  // one device index buffer for all launches
   std::vector<int> idx;
   std::vector<size_t> roff(launches.size()), coff(launches.size());
   for (size_t i = 0; i < launches.size(); ++i) {
@@ -117,89 +275,99 @@ void eri_run(Context &C, const EriRequest &R) {
     coff[i] = idx.size();
     idx.insert(idx.end(), launches[i].cols.begin(), launches[i].cols.end());
   }
-  DBuf<int> didx;
-  didx.upload(idx, C.stream);
-  // global workspace for the largest classes
-  size_t need = 0;
-  for (auto &L : launches) {
-    int l[4] = {std::get<0>(L.rk), std::get<1>(L.rk), std::get<0>(L.ck), std::get<1>(L.ck)};
-    int lc[4], m[4];
-    canonicalize(l, lc, m);
-    const ClassPlan *pl = class_plan(lc[0], lc[1], lc[2], lc[3]);
-    if (!pl) {
-      char buf[160];
-      snprintf(buf, sizeof buf, "no ERI kernel for class (%d%d|%d%d); shells with l > 3 are not supported",
-               lc[0], lc[1], lc[2], lc[3]);
-      throw Error{3, buf};
-    }
-    if (pl->mode == MODE_GLB) need = std::max(need, (size_t)2048 * pl->per_q);
-  }
-  if (need) C.gws.alloc(need);
+  C.md_idx.upload(idx, C.stream);
 
-  hipEvent_t e0 = nullptr, e1 = nullptr;
-  if (C.time_eri) {
-    GI_HIP(hipEventCreate(&e0));
-    GI_HIP(hipEventCreate(&e1));
-    GI_HIP(hipEventRecord(e0, C.stream));
-  }
-  long long nq_total = 0, flops_total = 0;
   static const bool stats = getenv("GIGPU_ERI_STATS") != nullptr;
+  long long nq_total = 0;
+  double flops_total = 0;
   for (size_t i = 0; i < launches.size(); ++i) {
     auto &L = launches[i];
-    int l[4] = {std::get<0>(L.rk), std::get<1>(L.rk), std::get<0>(L.ck), std::get<1>(L.ck)};
-    int np[4] = {std::get<2>(L.rk), std::get<3>(L.rk), std::get<2>(L.ck), std::get<3>(L.ck)};
-    int lc[4], m[4];
-    canonicalize(l, lc, m);
-    auto &ent = tab.at((lc[0] * 8 + lc[1]) * 64 + lc[2] * 8 + lc[3]);
-    EriArgs a{};
-    a.shells = C.dshells.p;
-    a.exps = C.dexps.p;
-    a.cntr = C.dcntr.p;
-    a.norms = C.dnorms.p;
-    a.boys = C.dboys.p;
-    a.rdesc = R.d_rdesc;
-    a.cdesc = R.d_cdesc;
-    a.rows = didx.p + roff[i];
-    a.cols = didx.p + coff[i];
-    a.nrows = (long long)L.rows.size();
-    a.ncols = (long long)L.cols.size();
+    const long long nrows = (long long)L.rows.size(), ncols = (long long)L.cols.size();
+    if (nrows == 0 || ncols == 0) continue;
+    // algorithmic operation count of the launch (useful rows/columns only)
+    const double NHE = nherm(L.E), NHF = nherm(L.F), rt = rtable_flops(L.E + L.F);
+    double fl = 0;
+    long long nq = 0;
+    auto info = [&](const PairDesc &d, double &nrow, double &np) {
+      nrow = pair_rows_host(d, C.shell_nang[d.s0], C.shell_nang[d.s1]);
+      np = (double)C.shell_np[d.s0] * C.shell_np[d.s1];
+    };
+    if (R.mode == LAUNCH_PRODUCT) {
+      double sr_np = 0, sr_nrnp = 0, sc_np = 0, sc_ncnp = 0, sc_nc = 0;
+      for (int r : L.rows) {
+        double n, p;
+        info(rd[r], n, p);
+        sr_np += p;
+        sr_nrnp += n * p;
+      }
+      for (int c : L.cols) {
+        double n, p;
+        info(cd[c], n, p);
+        sc_np += p;
+        sc_ncnp += n * p;
+        sc_nc += n;
+      }
+      // stage 1: 2 NHE NHF nc per primitive quartet; stage 2: 2 nr NHE nc per bra primitive pair
+      fl = 2.0 * NHE * NHF * sr_np * sc_ncnp + 2.0 * NHE * sr_nrnp * sc_nc + rt * sr_np * sc_np;
+      nq = nrows * ncols;
+    } else {
+      for (size_t k = 0; k < L.rows.size(); ++k) {
+        double nr, pr, nc, pc;
+        info(rd[L.rows[k]], nr, pr);
+        info(cd[L.cols[k]], nc, pc);
+        const double st2 = (R.mode == LAUNCH_DIAG) ? 2.0 * NHE * nc * pr : 2.0 * NHE * nr * nc * pr;
+        fl += 2.0 * NHE * NHF * nc * pr * pc + st2 + rt * pr * pc;
+      }
+      nq = nrows;
+    }
+    MdArgs a{};
+    a.hdr = C.md_hdr.p;
+    a.prim = C.md_prim.p;
+    a.et = C.md_et.p;
+    a.rdesc = C.md_rdesc.p;
+    a.cdesc = C.md_cdesc.p;
+    a.rows = C.md_idx.p + roff[i];
+    a.cols = C.md_idx.p + coff[i];
+    a.nrows = nrows;
+    a.ncols = ncols;
     a.mode = R.mode;
-    a.perm = m[0] | (m[1] << 2) | (m[2] << 4) | (m[3] << 6);
-    a.spher = C.spher;
+    a.kt = 1;
+    if (R.mode == LAUNCH_PRODUCT) {
+      const long long q = nrows * ncols;
+      a.kt = (int)std::max<long long>(1, std::min<long long>(4, q / 65536));
+      a.ntiles = nrows * ((ncols + a.kt - 1) / a.kt);
+    } else {
+      a.ntiles = nrows;
+    }
     a.kappa = R.kappa;
     a.out = R.out;
     a.ldo = R.ldo;
     a.sel = R.d_sel;
-    a.gws = C.gws.p;
-    a.npa = np[m[0]];
-    a.npb = np[m[1]];
-    a.npc = np[m[2]];
-    a.npd = np[m[3]];
-    long long ntask = (R.mode == LAUNCH_PRODUCT) ? a.nrows * a.ncols : a.nrows;
-    if (ntask == 0) continue;
-    long long ngroups = (ntask + ent.plan.qpw - 1) / ent.plan.qpw;
+    a.boys = C.dboys.p;
+    long long grid = std::min<long long>(a.ntiles, 4096);
+    if (grid >= 8) grid = (grid + 7) / 8 * 8;
     hipEvent_t s0 = nullptr, s1 = nullptr;
     if (stats) {
       GI_HIP(hipEventCreate(&s0));
       GI_HIP(hipEventCreate(&s1));
       GI_HIP(hipEventRecord(s0, C.stream));
     }
-    ent.fn(a, ngroups, C.stream);
+    md_table[L.E][L.F](a, (int)grid, C.stream);
     GI_HIP(hipGetLastError());
     if (stats) {
       GI_HIP(hipEventRecord(s1, C.stream));
       GI_HIP(hipEventSynchronize(s1));
       float ms = 0;
       GI_HIP(hipEventElapsedTime(&ms, s0, s1));
-      auto &st = C.class_stats[(lc[0] * 8 + lc[1]) * 64 + lc[2] * 8 + lc[3]];
-      st[0] += ntask;
+      auto &st = C.class_stats[L.E * 16 + L.F];
+      st[0] += (double)nq;
       st[1] += ms;
-      st[2] += (double)ntask * ent.plan.flops * (np[0] * np[1] * np[2] * np[3]);
+      st[2] += fl;
       (void)hipEventDestroy(s0);
       (void)hipEventDestroy(s1);
     }
-    nq_total += ntask;
-    flops_total += ntask * ent.plan.flops * (long long)(np[0] * np[1] * np[2] * np[3]);
+    nq_total += nq;
+    flops_total += fl;
   }
   if (C.time_eri) {
     GI_HIP(hipEventRecord(e1, C.stream));
@@ -211,8 +379,8 @@ void eri_run(Context &C, const EriRequest &R) {
     (void)hipEventDestroy(e1);
   }
   C.eri_quartets_acc += nq_total;
-  C.eri_flops_acc += flops_total;
-  // didx must outlive the kernels
+  C.eri_flops_acc += (long long)flops_total;
+  // the pair table and index buffers are reused by the next request
   GI_HIP(hipStreamSynchronize(C.stream));
 }
 

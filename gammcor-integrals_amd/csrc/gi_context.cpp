@@ -28,7 +28,13 @@ Context::~Context() {
   dcntr.release();
   dnorms.release();
   dboys.release();
-  gws.release();
+  md_hdr.release();
+  md_jobs.release();
+  md_prim.release();
+  md_et.release();
+  md_rdesc.release();
+  md_cdesc.release();
+  md_idx.release();
   chol.inv_l.release();
   chol.rkpq.release();
   if (stream) (void)hipStreamDestroy(stream);

@@ -6,7 +6,7 @@
 #include <map>
 #include <memory>
 #include <vector>
-#include "eri_kernel.h"
+#include "md_kernel.h"
 #include "gi_common.h"
 
 namespace gi {
@@ -39,12 +39,7 @@ struct DBuf {
   void upload(const std::vector<T> &v, hipStream_t s) { upload(v.data(), v.size(), s); }
 };
 
-struct ClassPlan {
-  int mode, G, qpw, per_q;
-  long long flops;  // FP64 operations per spherical quartet
-};
-const ClassPlan *class_plan(int la, int lb, int lc, int ld);   // canonical class
-bool class_supported(int la, int lb, int lc, int ld);
+constexpr int MD_EMAX = 6;   // largest l0 + l1 of a shell pair with an ERI kernel
 
 // Cholesky outputs kept on the device / host (TChol2Vecs + Rkpq)
 struct CholState {
@@ -76,15 +71,21 @@ struct Context {
   std::vector<int> shell_l, shell_nang, shell_np, shell_loc;   // shell_loc: 0-based first AO
   DBuf<DevShell> dshells;
   DBuf<double> dexps, dcntr, dnorms, dboys;
-  DBuf<double> gws;
   bool basis_set = false;
+  // pair tables and index lists of the last ERI request (reused, grow only)
+  DBuf<PairHdr> md_hdr;
+  DBuf<EtJob> md_jobs;
+  DBuf<PrimRec> md_prim;
+  DBuf<double> md_et;
+  DBuf<PairDesc> md_rdesc, md_cdesc;
+  DBuf<int> md_idx;
   CholState chol;
   // profiling of ERI launches (events on the library stream)
   bool time_eri = false;
   double eri_ms_acc = 0;
   long long eri_quartets_acc = 0;
   long long eri_flops_acc = 0;
-  // per canonical class (key (la*8+lb)*64+lc*8+ld): quartets, kernel ms, FP64 ops
+  // per kernel class (key 16*E + F, E = la+lb, F = lc+ld): quartets, kernel ms, FP64 ops
   // (filled when GIGPU_ERI_STATS is set in the environment)
   std::map<int, std::array<double, 3>> class_stats;
 
@@ -94,11 +95,10 @@ struct Context {
 
 // ---------------------------------------------------------------- ERI driver
 // Evaluate the quartets of a row-list x column-list product (or zipped lists)
-// into a column-major destination, binning pairs by angular momenta.
+// into a column-major destination, binning pairs by l0 + l1.
 struct EriRequest {
-  const std::vector<PairDesc> *rdesc = nullptr;   // host copies (for binning)
+  const std::vector<PairDesc> *rdesc = nullptr;   // host lists
   const std::vector<PairDesc> *cdesc = nullptr;
-  const PairDesc *d_rdesc = nullptr, *d_cdesc = nullptr;
   const std::vector<int> *rows = nullptr, *cols = nullptr;
   int mode = LAUNCH_PRODUCT;
   double *out = nullptr;

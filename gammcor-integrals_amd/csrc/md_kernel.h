@@ -1,0 +1,374 @@
+// Hermite-matrix McMurchie-Davidson ERI engine for CDNA4 (gfx950).
+//
+// A shell quartet block is the matrix product
+//   V(ab, cd) = sum_{h, k} E^AB(ab, h) (-1)^{|k|} R_{h+k}(alpha, P-Q) E^CD(cd, k)
+// with h, k Hermite triples (t,u,v), E^AB the two-centre Hermite expansion
+// coefficients of the bra pair (contraction coefficients, Gaussian product
+// factor, solid-harmonic transform and AO normalization folded in) and
+// R_{tuv} the Hermite Coulomb integrals (Helgaker, Joergensen, Olsen,
+// "Molecular Electronic-Structure Theory", eqs. 9.5.33, 9.9.18-9.9.20).  The
+// reference evaluates the same integrals through (e0|f0) blocks and the
+// Head-Gordon-Pople horizontal recurrence (auto2e_WMatrix_part1.f90:99-148,
+// Auto2e.py:298-338); the Boys function is evaluated the same way, once per
+// primitive quartet at top order L = la+lb+lc+ld, then recursed downwards
+// (boys.f90:202-281).
+//
+// Per shell pair the table ET(h, r) = E^AB(r, h) is built once per launch
+// (k_build_et), r = full orbital-pair index a + na*b.  Per quartet one
+// wavefront builds R_{h+k} in LDS and evaluates
+//   stage 1  X(h, c) = sum_k (-1)^{|k|} R_{h+k} ET_ket(k, c)      (MFMA f64 16x16x4)
+//   stage 2  V(r, c) = sum_h ET_bra(h, r) X(h, c)                (MFMA f64 16x16x4)
+// sixteen ket columns c at a time.  Rows and columns are the destination's
+// own orbital pairs (full block, same-shell lower triangle, or a selection
+// list), so the result tiles go straight to the chol2_write_V* layouts of
+// TwoStepCholesky_Step{1,2}.f90 with row-contiguous (coalesced) stores.
+#pragma once
+#include "gi_common.h"
+
+namespace gi {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+// One shell pair as the caller sees it.  The destination address of
+// element (jr, jc) of a quartet block is  out + row.off + col.off + jr + ldo * jc.
+struct PairDesc {
+  int s0, s1;       // shells (0-based), caller order
+  int flags;        // PAIR_SAME: fold to lower triangle; PAIR_SEL: selection list
+  int sel0, nsel;   // selection list slice: 0-based full-block indices a + na*b
+  int pt;           // pair-table entry (filled by the launch layer)
+  long long off;    // destination offset contributed by this pair
+};
+enum { PAIR_SAME = 1, PAIR_SEL = 2 };
+enum { LAUNCH_PRODUCT = 0, LAUNCH_ZIP = 1, LAUNCH_DIAG = 2 };
+
+// pair table
+struct PairHdr {
+  int n0, n1;       // functions per shell (spherical or Cartesian)
+  int nprim;        // primitive pairs
+  int prim0;        // first PrimRec
+};
+struct PrimRec {
+  double px, py, pz, p;
+  long long et;     // offset (doubles) of the ET block [nherm(l0+l1)][n0*n1]
+  long long pad;
+};
+struct EtJob {
+  int s0, s1, ia, ib;   // shells, primitive indices
+  int prim, pad;        // PrimRec slot
+  long long et;         // ET block offset
+};
+
+constexpr int LRMAX = 4 * MAXL;                 // highest R-table order
+constexpr int NHRMAX = (LRMAX + 1) * (LRMAX + 2) * (LRMAX + 3) / 6;
+
+// c_herm[idx]: packed (t,u,v) of Hermite index idx.
+// c_rrec[idx]: recurrence of R_{tuv}, idx >= 1: i1 | i2 << 11 | dir << 22 | coef << 24,
+//   R^(n)_{tuv} = coef R^(n+1)_{i2} + PQ_dir R^(n+1)_{i1}
+__constant__ extern int c_herm[NHRMAX];
+__constant__ extern int c_rrec[NHRMAX];
+__constant__ extern double c_spher[SPHER_TOTAL];
+
+__host__ __device__ constexpr int nsph(int l, int sph) { return (sph && l >= 2) ? 2 * l + 1 : ncart(l); }
+__host__ __device__ constexpr int imax(int a, int b) { return a > b ? a : b; }
+__host__ __device__ constexpr int imin(int a, int b) { return a < b ? a : b; }
+
+// ------------------------------------------------------------- Boys function
+// F_0..F_M at x: 5-point Chebyshev interpolation of F_M on the interval of x,
+// then downward recursion; x >= 25: 1/2 sqrt(pi/x) and upward recursion
+// (boys.f90:202-281, chebinterp.f90:122-149).
+template <int M>
+__device__ __forceinline__ void boys_fm(double x, const double *__restrict__ tab, double (&F)[M + 1]) {
+  constexpr double dx = BOYS_XASYM / (double)BOYS_NINT;
+  constexpr double inv_dx = (double)BOYS_NINT / BOYS_XASYM;
+  if (x < BOYS_XASYM) {
+    int k = (int)floor(x / dx);   // same interval choice as floor(x / DELTAX)
+    if (k > BOYS_NINT - 1) k = BOYS_NINT - 1;
+    if (k < 0) k = 0;
+    const double *c = tab + 5 * (k + BOYS_NINT * M);
+    double ab_sum = (double)(2 * k + 1) * dx;
+    double y = (2.0 * x - ab_sum) * inv_dx;
+    double twoy = 2.0 * y;
+    double d5 = c[4];
+    double d4v = twoy * d5 + c[3];
+    double d3 = twoy * d4v - d5 + c[2];
+    double d2 = twoy * d3 - d4v + c[1];
+    double f = y * d2 - d3 + 0.5 * c[0];
+    F[M] = f;
+    double expmx = exp(-x), twox = 2.0 * x;
+#pragma unroll
+    for (int l = M; l >= 1; --l) {
+      f = (twox * f + expmx) * (1.0 / (double)(2 * l - 1));
+      F[l - 1] = f;
+    }
+  } else {
+    double f = 0.5 * sqrt(3.141592653589793 / x);
+    F[0] = f;
+    double expmx = exp(-x), beta = 1.0 / (2.0 * x);
+#pragma unroll
+    for (int l = 1; l <= M; ++l) {
+      f = beta * ((double)(2 * l - 1) * f - expmx);
+      F[l] = f;
+    }
+  }
+}
+
+// ------------------------------------------------------------- kernel shape
+__host__ __device__ constexpr int nab_max(int E) {   // largest Cartesian pair block, l <= MAXL
+  int m = 1;
+  for (int l0 = 0; l0 <= MAXL; ++l0)
+    for (int l1 = 0; l1 <= MAXL; ++l1)
+      if (l0 + l1 == E) m = imax(m, ncart(l0) * ncart(l1));
+  return m;
+}
+
+template <int E_, int F_>
+struct MdShape {
+  static constexpr int E = E_, F = F_, L = E + F;
+  static constexpr int NHE = nherm(E), NHF = nherm(F), NHL = nherm(L);
+  static constexpr int HT = (NHE + 15) / 16;    // stage-1 row tiles (h)
+  static constexpr int KSF = (NHF + 3) / 4;     // stage-1 k steps
+  static constexpr int KSE = (NHE + 3) / 4;     // stage-2 k steps
+  static constexpr int RT = (nab_max(E) + 15) / 16;   // stage-2 column tiles (bra rows)
+  static constexpr int NIDX = KSF * HT * 64;
+};
+
+struct MdArgs {
+  const PairHdr *hdr;
+  const PrimRec *prim;
+  const double *et;
+  const PairDesc *rdesc, *cdesc;
+  const int *rows, *cols;        // indices into rdesc / cdesc
+  long long nrows, ncols, ntiles;
+  int mode, kt;                  // LAUNCH_*, ket pairs per tile (PRODUCT)
+  double kappa;
+  double *out;
+  long long ldo;
+  const int *sel;                // selection lists
+  const double *boys;            // (5, BOYS_NINT, 0:BOYS_MAXM)
+};
+
+// rows of a pair in the destination, and the full-block index of row r
+__device__ __forceinline__ int pair_nrow(const PairDesc &d, const PairHdr &h) {
+  return (d.flags & PAIR_SEL) ? d.nsel : ((d.flags & PAIR_SAME) ? h.n0 * (h.n0 + 1) / 2 : h.n0 * h.n1);
+}
+__device__ __forceinline__ int pair_full(const PairDesc &d, const PairHdr &h, const int *sel, int r) {
+  if (d.flags & PAIR_SEL) return sel[d.sel0 + r];
+  if (d.flags & PAIR_SAME) {
+    // column-major lower triangle position r -> (a, b), a >= b
+    const int n = h.n0;
+    int b = 0, rem = r;
+    while (rem >= n - b) { rem -= n - b; ++b; }
+    return (b + rem) + n * b;
+  }
+  return r;
+}
+
+// R_{tuv} (times 2 pi^{5/2} / (p q sqrt(p + q + kappa p q))) for one primitive
+// quartet into Ra (level 0); Rb is the ping-pong partner.  Both hold a zero in
+// slot NHL.  Ends with a barrier.
+template <int L>
+__device__ __forceinline__ void rtable(double *Ra, double *Rb, const PrimRec &pb, const PrimRec &pk,
+                                       double kappa, const double *__restrict__ boys, int lane) {
+  const double PQx = pb.px - pk.px, PQy = pb.py - pk.py, PQz = pb.pz - pk.pz;
+  const double pq = pb.p * pk.p;
+  const double den = pb.p + pk.p + kappa * pq;
+  const double alpha = pq / den;
+  const double N = 34.986836655249725 / (pq * sqrt(den));   // 2 pi^(5/2)
+  double Fm[L + 1];
+  boys_fm<L>(alpha * (PQx * PQx + PQy * PQy + PQz * PQz), boys, Fm);
+  const double m2a = -2.0 * alpha;
+  double pw = N;
+#pragma unroll
+  for (int n = 0; n <= L; ++n) {
+    Fm[n] *= pw;
+    pw *= m2a;
+  }
+  __syncthreads();   // previous readers of Ra/Rb are done
+  {
+    double *top = (L % 2 == 0) ? Ra : Rb;
+    if (lane == 0) top[0] = Fm[L];
+  }
+#pragma unroll
+  for (int n = L - 1; n >= 0; --n) {
+    double *cur = (n % 2 == 0) ? Ra : Rb;
+    const double *prv = (n % 2 == 0) ? Rb : Ra;
+    __syncthreads();
+    const int cnt = nherm(L - n);
+#pragma unroll
+    for (int base = 0; base < nherm(L); base += 64) {
+      const int idx = base + lane;
+      if (base < cnt && idx < cnt) {
+        double r;
+        if (idx == 0) {
+          r = Fm[n];
+        } else {
+          const int rec = c_rrec[idx];
+          const int i1 = rec & 0x7ff, i2 = (rec >> 11) & 0x7ff, dir = (rec >> 22) & 3, coef = rec >> 24;
+          const double pqd = dir == 0 ? PQx : (dir == 1 ? PQy : PQz);
+          r = (double)coef * prv[i2] + pqd * prv[i1];
+        }
+        cur[idx] = r;
+      }
+    }
+  }
+  __syncthreads();
+}
+
+// ------------------------------------------------------------- quartet kernel
+template <int E, int F>
+__global__ void __launch_bounds__(64) md_kernel(MdArgs A) {
+  using S = MdShape<E, F>;
+  constexpr int HT = S::HT, KSF = S::KSF, KSE = S::KSE, RT = S::RT, NHE = S::NHE, NHF = S::NHF;
+  constexpr int NHL = S::NHL;
+  __shared__ unsigned short s_idx[S::NIDX];
+  __shared__ double s_R[2][NHL + 1];
+  __shared__ double s_X[HT * 16 * 16];
+
+  const int lane = threadIdx.x;
+  const int i16 = lane & 15, g = lane >> 4;
+  // stage-1 operand map: lane (i, kk) of step ks, row tile ht reads
+  // (-1)^{|k|} R_{h+k}, h = 16 ht + i, k = 4 ks + kk; slot NHL (zero) pads.
+  for (int e = lane; e < S::NIDX; e += 64) {
+    const int ln = e & 63, blk = e >> 6;
+    const int ks = blk / HT, ht = blk - ks * HT;
+    const int h = 16 * ht + (ln & 15), k = 4 * ks + (ln >> 4);
+    unsigned short v = (unsigned short)NHL;
+    if (h < NHE && k < NHF) {
+      const int th = c_herm[h], tk = c_herm[k];
+      const int t = px(th) + px(tk), u = py(th) + py(tk), w = pz(th) + pz(tk);
+      v = (unsigned short)hidx(t, u, w);
+      if ((px(tk) + py(tk) + pz(tk)) & 1) v |= 0x8000;
+    }
+    s_idx[e] = v;
+  }
+  if (lane == 0) {
+    s_R[0][NHL] = 0.0;
+    s_R[1][NHL] = 0.0;
+  }
+  __syncthreads();
+
+  // XCD-aware order: consecutive tiles (same bra pair) on one XCD's L2
+  const int G = gridDim.x;
+  const int w = blockIdx.x;
+  const long long v0 = (G % 8 == 0) ? (long long)(w % 8) * (G / 8) + w / 8 : w;
+  const long long nct = (A.mode == LAUNCH_PRODUCT) ? (A.ncols + A.kt - 1) / A.kt : 1;
+
+  for (long long t = v0; t < A.ntiles; t += G) {
+    long long ir, jc0;
+    int nk;
+    if (A.mode == LAUNCH_PRODUCT) {
+      ir = t / nct;
+      jc0 = (t - ir * nct) * A.kt;
+      nk = (int)((A.ncols - jc0) < A.kt ? (A.ncols - jc0) : A.kt);
+    } else {
+      ir = t;
+      jc0 = t;
+      nk = 1;
+    }
+    const PairDesc RD = A.rdesc[A.rows[ir]];
+    const PairHdr BH = A.hdr[RD.pt];
+    const int nr = pair_nrow(RD, BH);
+    const int ldb = BH.n0 * BH.n1;
+    int fr[RT];
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt) {
+      const int r = 16 * rt + i16;
+      fr[rt] = pair_full(RD, BH, A.sel, r < nr ? r : nr - 1);
+    }
+    const int nrt = (nr + 15) / 16;
+
+    for (int kq = 0; kq < nk; ++kq) {
+      const PairDesc CD = A.cdesc[A.cols[jc0 + kq]];
+      const PairHdr KH = A.hdr[CD.pt];
+      const int nc = pair_nrow(CD, KH);
+      const int ldk = KH.n0 * KH.n1;
+      double *dst = A.out + RD.off + CD.off;
+      const bool single = (BH.nprim == 1 && KH.nprim == 1);
+      if (single) rtable<E + F>(s_R[0], s_R[1], A.prim[BH.prim0], A.prim[KH.prim0], A.kappa, A.boys, lane);
+
+      for (int c0 = 0; c0 < nc; c0 += 16) {
+        const int cc = c0 + i16;
+        const int fc = pair_full(CD, KH, A.sel, cc < nc ? cc : nc - 1);
+        d4 vacc[RT];
+#pragma unroll
+        for (int rt = 0; rt < RT; ++rt) vacc[rt] = d4{0.0, 0.0, 0.0, 0.0};
+        double dacc = 0.0;
+
+        for (int bp = 0; bp < BH.nprim; ++bp) {
+          const PrimRec PB = A.prim[BH.prim0 + bp];
+          d4 xacc[HT];
+#pragma unroll
+          for (int ht = 0; ht < HT; ++ht) xacc[ht] = d4{0.0, 0.0, 0.0, 0.0};
+          for (int kp = 0; kp < KH.nprim; ++kp) {
+            const PrimRec PK = A.prim[KH.prim0 + kp];
+            if (!single) rtable<E + F>(s_R[0], s_R[1], PB, PK, A.kappa, A.boys, lane);
+            const double *etk = A.et + PK.et;
+            const double *Rf = s_R[0];
+#pragma unroll
+            for (int ks = 0; ks < KSF; ++ks) {
+              const int k = 4 * ks + g;
+              const double b = (k < NHF) ? etk[(long long)k * ldk + fc] : 0.0;
+#pragma unroll
+              for (int ht = 0; ht < HT; ++ht) {
+                const unsigned short e = s_idx[(ks * HT + ht) * 64 + lane];
+                double a = Rf[e & 0x7fff];
+                a = (e & 0x8000) ? -a : a;
+                xacc[ht] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, xacc[ht], 0, 0, 0);
+              }
+            }
+          }
+          // X(h, c) to LDS.  MFMA f64 16x16x4 result layout: element (i, j) of a
+          // tile sits in lane j + 16 (i % 4), register i / 4, so lane (j, g)
+          // holds rows 16 ht + 4 q + g of column j.
+          __syncthreads();
+#pragma unroll
+          for (int ht = 0; ht < HT; ++ht)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) s_X[(16 * ht + 4 * q + g) * 16 + i16] = xacc[ht][q];
+          __syncthreads();
+          const double *etb = A.et + PB.et;
+          if (A.mode == LAUNCH_DIAG) {
+            // diagonal elements only: V(c, c) = sum_h ET_bra(h, c) X(h, c)
+            if (g == 0) {
+              double s = 0.0;
+#pragma unroll 4
+              for (int h = 0; h < NHE; ++h) s += etb[(long long)h * ldb + fc] * s_X[h * 16 + i16];
+              dacc += s;
+            }
+          } else {
+#pragma unroll
+            for (int ks = 0; ks < KSE; ++ks) {
+              const int h = 4 * ks + g;
+              const double a = s_X[h * 16 + i16];
+#pragma unroll
+              for (int rt = 0; rt < RT; ++rt) {
+                if (rt < nrt) {
+                  const double b = (h < NHE) ? etb[(long long)h * ldb + fr[rt]] : 0.0;
+                  vacc[rt] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, vacc[rt], 0, 0, 0);
+                }
+              }
+            }
+          }
+        }
+        // store: lane (j, g) holds V(16 rt + j, c0 + 4 q + g)
+        if (A.mode == LAUNCH_DIAG) {
+          if (g == 0 && cc < nc) dst[cc] = dacc;
+        } else {
+#pragma unroll
+          for (int rt = 0; rt < RT; ++rt) {
+            const int r = 16 * rt + i16;
+            if (rt < nrt && r < nr) {
+#pragma unroll
+              for (int q = 0; q < 4; ++q) {
+                const int c = c0 + 4 * q + g;
+                if (c < nc) dst[r + A.ldo * c] = vacc[rt][q];
+              }
+            }
+          }
+        }
+      }
+    }
+  }
+}
+
+}  // namespace gi

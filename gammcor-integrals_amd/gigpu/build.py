@@ -27,7 +27,7 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH, "-Wno-pass-fai
 LIBS = ["-L/opt/rocm/lib", "-lrocblas", "-lrocsolver", "-Wl,-rpath,/opt/rocm/lib"]
 
 SOURCES = ["eri.hip", "chol.hip", "mo.hip", "gi_tables.cpp", "gi_context.cpp", "sort.cpp", "capi.cpp"]
-HEADERS = ["gi_common.h", "gi_context.h", "eri_kernel.h", "eri_classes.inc"]
+HEADERS = ["gi_common.h", "gi_context.h", "md_kernel.h", "md_classes.inc"]
 
 
 def _compile(src: str, incremental: bool) -> str:
@@ -46,7 +46,7 @@ def _compile(src: str, incremental: bool) -> str:
 
 def build(incremental: bool = False, jobs: int = 8) -> str:
     sys.path.insert(0, PKG)
-    from gigpu import eri_gen  # noqa: E402  (emits csrc/eri_classes.inc)
+    from gigpu import eri_gen  # noqa: E402  (emits csrc/md_classes.inc)
     eri_gen.emit()
     os.makedirs(BUILD, exist_ok=True)
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:

@@ -190,11 +190,11 @@ class Context:
         return dict(zip(keys, t.tolist()))
 
     def eri_class_stats(self):
-        """[(la, lb, lc, ld, quartets, kernel_ms, fp64_ops)] (needs GIGPU_ERI_STATS)."""
-        rows = np.zeros(7 * 512)
+        """[(E, F, quartets, kernel_ms, fp64_ops)] per kernel (la+lb, lc+ld) (needs GIGPU_ERI_STATS)."""
+        rows = np.zeros(5 * 512)
         n = C.c_int64()
         _check(self.L.gi_eri_class_stats(self.h, 512, _p(rows), C.byref(n)))
-        return [tuple(rows[7 * k:7 * k + 7]) for k in range(n.value)]
+        return [tuple(rows[5 * k:5 * k + 5]) for k in range(n.value)]
 
     def mo_transform(self, CA, a0, a1, CB, b0, b1, max_buffer_mb=1 << 20, flags=0):
         CA = np.asfortranarray(CA, dtype=np.float64)

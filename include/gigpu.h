@@ -116,10 +116,10 @@ int gi_chol_vdiag(gi_context *ctx, double *dst);
  * [13] Inv_L V^T GEMM. */
 int gi_chol_timings(gi_context *ctx, double t[14]);
 
-/* Per-class ERI statistics (diagnostic; no reference counterpart), gathered
+/* Per-kernel ERI statistics (diagnostic; no reference counterpart), gathered
  * when the environment variable GIGPU_ERI_STATS is set: up to `cap` rows of
- * (la, lb, lc, ld, quartets, kernel milliseconds, FP64 operations), canonical
- * classes, accumulated since the context was created.  *n = rows written. */
+ * (E = la+lb, F = lc+ld, quartets, kernel milliseconds, FP64 operations),
+ * accumulated since the context was created.  *n = rows written. */
 int gi_eri_class_stats(gi_context *ctx, int64_t cap, double *rows, int64_t *n);
 
 /* AO -> MO transform of the Cholesky vectors, chol_gammcor_MOTransf_2

@@ -171,6 +171,23 @@ class CholResult:
                             rtr=r.t_rtr, rk=r.t_rk, compress=r.t_compress, vpqrs=r.t_vpqrs,
                             cholesky=r.t_chol, inverse=r.t_inv)
 
+    def set_rkpq(self, rkpq):
+        """Replace the vectors the oracle's MO transform reads (same shape as
+        self.rkpq), so that the transform can be checked on vectors produced
+        elsewhere."""
+        r = self.res
+        n = r.nvecs * r.max_subset_dim * r.nsubsets
+        rk = np.ascontiguousarray(rkpq, dtype=np.float64).reshape(-1)
+        if not r.rkpq or rk.size != n:
+            raise ValueError("rkpq shape does not match this decomposition")
+        np.ctypeslib.as_array(r.rkpq, shape=(n,))[:] = rk
+        self.rkpq = rk.reshape(r.nsubsets, r.max_subset_dim, r.nvecs).copy()
+
+    def full_vectors(self):
+        """Cholesky vectors as a dense (nvecs, norb_pairs) array in FULL
+        storage order (see full_vectors())."""
+        return full_vectors(self.shell_pair_loc, self.shell_pair_dim, self.rkpq, self.norb_pairs)
+
     def mo_transform(self, CA, a0, a1, CB, b0, b1, max_buffer_mb=100000):
         """Rkab (nvecs, NA*NB) in Fortran layout, returned as (NA*NB, nvecs)
         C-order view transposed back to (nvecs, NA*NB)."""
@@ -191,6 +208,49 @@ class CholResult:
             self.oracle.L.or_chol_free(C.byref(self.res))
         except Exception:
             pass
+
+
+def full_vectors(shell_pair_loc, shell_pair_dim, rkpq, norb_pairs):
+    """(nvecs, norb_pairs) array R[k, f] = R_k at FULL location f + 1.
+    shell_pair_loc[s] = (FULL, subset storage, subset index), 1-based, as in
+    ShellPairLoc (TwoStepCholesky_definitions.f90:56-74); rkpq[X, i, k] is
+    Rkpq(k, i + 1, X + 1)."""
+    nvecs = rkpq.shape[2]
+    R = np.zeros((nvecs, norb_pairs))
+    for s in range(len(shell_pair_dim)):
+        f0, i0, X = (int(v) for v in shell_pair_loc[s])
+        d = int(shell_pair_dim[s])
+        R[:, f0 - 1:f0 - 1 + d] = rkpq[X - 1, i0 - 1:i0 - 1 + d, :].T
+    return R
+
+
+def step2_columns(basis, meta):
+    """0-based FULL locations of the orbital pairs Step 2 factorizes (the
+    decoded pivots: PivotOrbPairs inside PivotShellPairs), in pivot order.
+    The vectors reproduce exactly these columns of V."""
+    sp = meta["shell_pairs"]
+    loc = meta["shell_pair_loc"]
+    psp = meta["pivot_shell_pairs"]
+    psploc = meta["pivot_shell_pair_loc"]
+    pspdim = meta["pivot_shell_pair_dim"]
+    porb = meta["pivot_orb_pairs"]
+    cols = []
+    for k in range(len(psp)):
+        ab = int(psp[k]) - 1
+        A, B = sp[ab]
+        na = basis.shell_nang(A - 1)
+        f0 = int(loc[ab][0]) - 1
+        for i in range(int(psploc[k]) - 1, int(psploc[k]) - 1 + int(pspdim[k])):
+            fb = int(porb[i]) - 1
+            a, b = fb % na, fb // na
+            if A == B:
+                if a < b:
+                    a, b = b, a
+                pos = sum(na - bb for bb in range(b)) + (a - b)
+            else:
+                pos = fb
+            cols.append(f0 + pos)
+    return np.array(cols, dtype=np.int64)
 
 
 def boys_fm(m, x):
