@@ -79,23 +79,26 @@ __host__ __device__ static inline double qabs(double x, double inv_delta) { retu
 // ---- micro iterations restricted to the batch rows -------------------------
 // The pivot of every micro iteration is chosen among the batch elements, and
 // its row of L is a batch row, so the sequential part of chol2_Pivots only
-// needs L and D on the batch rows.  The loop runs on the device, one fused
-// kernel per micro iteration: column update of the batch rows, diagonal
-// update, and the first-maximum search finished by the last workgroup to
-// arrive (no host round trip per iteration).  The remaining candidate rows
-// are filled afterwards in one triangular solve (see chol_build).
+// needs L and D on the batch rows (at most 1000 of them: the batch limit of
+// chol2_DefineBatch).  The loop runs on the device in panels of MICRO_PANEL
+// steps: one workgroup of 1024 lanes (one lane per batch row) runs a panel
+// with the panel's columns in LDS, computing each column from the batch block
+// A, which holds M(batch, batch) minus the contributions of all earlier
+// panels; k_micro_update then subtracts the panel's rank-MICRO_PANEL term
+// from A.  The first-maximum search stays inside the workgroup, so a panel is
+// one launch with no host round trip.  The remaining candidate rows are
+// filled afterwards in one triangular solve (see chol_build).
+constexpr int MICRO_PANEL = 16;
+constexpr int MICRO_ROWS = 1024;
+
 struct MicroState {
   int done;       // loop has stopped
   int j;          // columns computed in this macro iteration
   int e;          // batch index of the current pivot
-  unsigned counter;
+  int p0;         // first column of the current panel
   double val;     // raw residual diagonal of the current pivot
 };
 
-struct MicroPartial {
-  double key, val;
-  long long idx;
-};
 
 __device__ __forceinline__ void argmax_merge(double &k1, double &v1, long long &i1, double k2, double v2,
                                              long long i2) {
@@ -139,9 +142,9 @@ __global__ void __launch_bounds__(1024) k_micro_init(const double *__restrict__ 
   block_argmax<1024>(bk, bv, bi);
   if (threadIdx.x == 0) {
     st->j = 0;
+    st->p0 = 0;
     st->e = (int)bi;
     st->val = bv;
-    st->counter = 0;
     st->done = (bi < 0 || nsteps <= 0) ? 1 : 0;
   }
 }
@@ -153,56 +156,102 @@ __global__ void k_gather_block(double *__restrict__ Mb, const double *__restrict
   if (e < nb && c < nb) Mb[e + nb * c] = M[bd[e] + ldM * c];
 }
 
-template <int BS>
-__global__ void __launch_bounds__(BS) k_micro_step(MicroState *st, const double *__restrict__ Mb,
-                                                   double *__restrict__ Lb, double *__restrict__ Db, long long nb,
-                                                   int nsteps, double dmin, double inv_delta,
-                                                   MicroPartial *__restrict__ part, int *__restrict__ piv) {
-  if (st->done) return;   // uniform: written by the previous launch
-  const int j = st->j, e = st->e;
-  const double val = st->val;
-  const double sq = sqrt(val);
-  const long long i = blockIdx.x * (long long)BS + threadIdx.x;
-  double bk = inv_delta > 0.0 ? qabs(-1.0, inv_delta) : -1.0, bv = -1.0;
-  long long bi = -1;
-  if (i < nb) {
-    double s = Mb[i + nb * e];
-    for (int q = 0; q < j; ++q) s -= Lb[i + nb * q] * Lb[e + nb * q];
-    const double l = s / sq;
-    Lb[i + nb * j] = l;
-    const double d = Db[i] - l * l;
-    Db[i] = d;
-    bk = inv_delta > 0.0 ? qabs(d, inv_delta) : d;
-    bv = d;
-    bi = i;
+// first maximum over a 1024-lane workgroup (key descending, index ascending;
+// the order of merging does not change the result).  Ends with a barrier.
+__device__ void wg_argmax_1024(double &k, double &v, int &i) {
+  __shared__ double sk[16], sv[16];
+  __shared__ int si[16];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const double k2 = __shfl_xor(k, off), v2 = __shfl_xor(v, off);
+    const int i2 = __shfl_xor(i, off);
+    const bool take = (i2 >= 0) && (i < 0 || k2 > k || (k2 == k && i2 < i));
+    if (take) { k = k2; v = v2; i = i2; }
   }
-  block_argmax<BS>(bk, bv, bi);
-  __shared__ unsigned ticket;
-  if (threadIdx.x == 0) {
-    part[blockIdx.x] = MicroPartial{bk, bv, bi};
-    __threadfence();
-    ticket = atomicAdd(&st->counter, 1u);
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (lane == 0) { sk[w] = k; sv[w] = v; si[w] = i; }
+  __syncthreads();
+  if (w == 0) {
+    if (lane < 16) { k = sk[lane]; v = sv[lane]; i = si[lane]; }
+    else { i = -1; }
+#pragma unroll
+    for (int off = 8; off > 0; off >>= 1) {
+      const double k2 = __shfl_xor(k, off), v2 = __shfl_xor(v, off);
+      const int i2 = __shfl_xor(i, off);
+      const bool take = (i2 >= 0) && (i < 0 || k2 > k || (k2 == k && i2 < i));
+      if (take) { k = k2; v = v2; i = i2; }
+    }
+    if (lane == 0) { sk[0] = k; sv[0] = v; si[0] = i; }
   }
   __syncthreads();
-  if (ticket != gridDim.x - 1) return;
-  // last workgroup of this step: finish the search, advance the state
-  __threadfence();
-  double k = inv_delta > 0.0 ? qabs(-1.0, inv_delta) : -1.0, v = -1.0;
-  long long ix = -1;
-  for (unsigned b = threadIdx.x; b < gridDim.x; b += BS) {
-    MicroPartial p = part[b];
-    argmax_merge(k, v, ix, p.key, p.val, p.idx);
+  k = sk[0]; v = sv[0]; i = si[0];
+  __syncthreads();
+}
+
+// One panel of micro iterations (chol2_Lj + D update + BatchMax,
+// TwoStepCholesky_Step1.f90:1027-1064) on the batch rows, nb <= MICRO_ROWS:
+//   L(i, j) = (A(i, e) - sum_{p0 <= q < j} L(i, q) L(e, q)) / sqrt(D(e)),
+//   D(i) -= L(i, j)^2,  e <- first maximum of D (quantized keys).
+__global__ void __launch_bounds__(MICRO_ROWS) k_micro_panel(MicroState *st, const double *__restrict__ A,
+                                                            double *__restrict__ Lb, double *__restrict__ Db,
+                                                            long long nb, int nsteps, double dmin, double inv_delta,
+                                                            int *__restrict__ piv) {
+  __shared__ double lp[MICRO_PANEL][MICRO_ROWS];
+  __shared__ int s_done;
+  if (threadIdx.x == 0) s_done = st->done;
+  __syncthreads();
+  if (s_done) return;
+  const int i = threadIdx.x;
+  const bool row = i < nb;
+  int j = st->j, e = st->e;
+  double val = st->val;
+  const int p0 = j;
+  int done = 0;
+  for (int step = 0; step < MICRO_PANEL; ++step) {
+    const double sq = sqrt(val);
+    double bk = inv_delta > 0.0 ? qabs(-1.0, inv_delta) : -1.0, bv = -1.0;
+    int bi = -1;
+    if (row) {
+      double s = A[i + nb * e];
+      for (int q = 0; q < step; ++q) s -= lp[q][i] * lp[q][e];
+      const double l = s / sq;
+      lp[step][i] = l;
+      Lb[i + nb * j] = l;
+      const double d = Db[i] - l * l;
+      Db[i] = d;
+      bk = inv_delta > 0.0 ? qabs(d, inv_delta) : d;
+      bv = d;
+      bi = i;
+    }
+    wg_argmax_1024(bk, bv, bi);   // its barriers also publish the panel column
+    if (threadIdx.x == 0) piv[j] = e;
+    ++j;
+    e = bi;
+    val = bv;
+    done = (bi < 0 || bv <= dmin || j >= nb || j >= nsteps) ? 1 : 0;
+    if (done) break;
   }
-  block_argmax<BS>(k, v, ix);
   if (threadIdx.x == 0) {
-    piv[j] = e;
-    const int jn = j + 1;
-    st->j = jn;
-    st->e = (int)ix;
-    st->val = v;
-    st->counter = 0;
-    st->done = (ix < 0 || v <= dmin || jn >= nb || jn >= nsteps) ? 1 : 0;
+    st->p0 = p0;
+    st->j = j;
+    st->e = e;
+    st->val = val;
+    st->done = done;
   }
+}
+
+// A(i, c) -= sum_{q = p0}^{j-1} L(i, q) L(c, q) for the panel just finished
+// (skipped once the loop has stopped: A is not read again).
+__global__ void k_micro_update(const MicroState *st, double *__restrict__ A, const double *__restrict__ Lb,
+                               long long nb) {
+  if (st->done) return;
+  const int p0 = st->p0, j = st->j;
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  const long long c = blockIdx.y;
+  if (i >= nb || c >= nb) return;
+  double s = A[i + nb * c];
+  for (int q = p0; q < j; ++q) s -= Lb[i + nb * q] * Lb[c + nb * q];
+  A[i + nb * c] = s;
 }
 
 // Lt(:, j0+q) = M(:, piv(q))  (right-hand sides of the triangular solve)
@@ -470,7 +519,6 @@ void chol_build(Context &C, double tau, double kappa, long long max_block_dim, l
     DBuf<double> dD, dDn, dL, dLn, dM, dLB, dtmp, dbm, dMb, dLb, dDb, dLP;
     DBuf<int> dpiv;
     DBuf<MicroState> dstate;
-    DBuf<MicroPartial> dpart;
     dD.upload(Dh.data(), nop, C.stream);
     long long ncand = nop;       // rows of L / length of D
     long long capL = 0;          // columns allocated in L
@@ -632,17 +680,26 @@ void chol_build(Context &C, double tau, double kappa, long long max_block_dim, l
         dDb.alloc(nb);
         dpiv.alloc(nb);
         dstate.alloc(1);
-        const int BS = 256;
-        const int nblk = (int)((nb + BS - 1) / BS);
-        dpart.alloc(nblk);
+        if (nb > MICRO_ROWS) throw Error{21, "batch larger than the micro-iteration workgroup"};
         hipLaunchKernelGGL(k_gather_block, dim3((unsigned)((nb + 255) / 256), (unsigned)nb), dim3(256), 0, C.stream,
                            dMb.p, dM.p, ncand, dbd.p, nb);
         const double qd = raw_ties ? 0.0 : inv_delta;
         hipLaunchKernelGGL(k_micro_init, dim3(1), dim3(1024), 0, C.stream, dD.p, dbd.p, nb, qd, dDb.p, dstate.p,
                            nsteps);
-        for (int s = 0; s < nsteps; ++s)
-          hipLaunchKernelGGL(k_micro_step<256>, dim3(nblk), dim3(BS), 0, C.stream, dstate.p, dMb.p, dLb.p, dDb.p,
-                             nb, nsteps, dmin, qd, dpart.p, dpiv.p);
+        // panels until the loop stops; the state is read back every 4 panels
+        const int npanel_max = (nsteps + MICRO_PANEL - 1) / MICRO_PANEL;
+        for (int pnl = 0; pnl < npanel_max;) {
+          for (int k = 0; k < 4 && pnl < npanel_max; ++k, ++pnl) {
+            hipLaunchKernelGGL(k_micro_panel, dim3(1), dim3(MICRO_ROWS), 0, C.stream, dstate.p, dMb.p, dLb.p,
+                               dDb.p, nb, nsteps, dmin, qd, dpiv.p);
+            hipLaunchKernelGGL(k_micro_update, dim3((unsigned)((nb + 255) / 256), (unsigned)nb), dim3(256), 0,
+                               C.stream, dstate.p, dMb.p, dLb.p, nb);
+          }
+          MicroState hs{};
+          GI_HIP(hipMemcpyAsync(&hs, dstate.p, sizeof(hs), hipMemcpyDeviceToHost, C.stream));
+          GI_HIP(hipStreamSynchronize(C.stream));
+          if (hs.done) break;
+        }
         MicroState hs{};
         GI_HIP(hipMemcpyAsync(&hs, dstate.p, sizeof(hs), hipMemcpyDeviceToHost, C.stream));
         GI_HIP(hipStreamSynchronize(C.stream));
@@ -848,11 +905,19 @@ void chol_build(Context &C, double tau, double kappa, long long max_block_dim, l
       }
       t_vab += now() - tv;
       double tg = now();
+      // Inv_L is lower triangular: block row r of Rkpq only needs the first
+      // r1 columns of Inv_L, so the product is one GEMM per block row
+      // ((nblk + 1) / (2 nblk) of the dense FLOPs)
       const double one = 1.0, zero = 0.0;
-      ROCBLAS_CHECK(rocblas_dgemm_strided_batched(
-          C.blas, rocblas_operation_none, rocblas_operation_transpose, nvec, msd, nvec, &one, S.inv_l.p, nvec, 0,
-          dW.p, msd, (rocblas_stride)slab, &zero, S.rkpq.p + (size_t)(X0 - 1) * nvec * msd, nvec,
-          (rocblas_stride)((size_t)nvec * msd), ng));
+      const long long nblk = std::max<long long>(1, std::min<long long>(8, nvec / 512));
+      for (long long rb = 0; rb < nblk; ++rb) {
+        const long long r0 = nvec * rb / nblk, r1 = nvec * (rb + 1) / nblk;
+        if (r1 <= r0) continue;
+        ROCBLAS_CHECK(rocblas_dgemm_strided_batched(
+            C.blas, rocblas_operation_none, rocblas_operation_transpose, r1 - r0, msd, r1, &one, S.inv_l.p + r0,
+            nvec, 0, dW.p, msd, (rocblas_stride)slab, &zero, S.rkpq.p + (size_t)(X0 - 1) * nvec * msd + r0, nvec,
+            (rocblas_stride)((size_t)nvec * msd), ng));
+      }
       GI_HIP(hipStreamSynchronize(C.stream));
       t_gemm += now() - tg;
     }
