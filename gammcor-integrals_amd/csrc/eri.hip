@@ -128,14 +128,22 @@ static void launch_md(const MdArgs &a, int grid, hipStream_t s) {
   hipLaunchKernelGGL((md_kernel<E, F>), dim3((unsigned)grid), dim3(64), 0, s, a);
 }
 
+template <int E, int F>
+static void launch_lane(const MdArgs &a, int grid, hipStream_t s) {
+  hipLaunchKernelGGL((md_lane_kernel<E, F>), dim3((unsigned)grid), dim3(256), 0, s, a);
+}
+
 static LaunchFn md_table[MD_EMAX + 1][MD_EMAX + 1];
+static LaunchFn lane_table[LANE_EMAX + 1][LANE_EMAX + 1];
 
 static void init_md_table() {
   static bool done = false;
   if (done) return;
 #define GI_MD(e, f) md_table[e][f] = &launch_md<e, f>;
+#define GI_LANE(e, f) lane_table[e][f] = &launch_lane<e, f>;
 #include "md_classes.inc"
 #undef GI_MD
+#undef GI_LANE
   done = true;
 }
 
@@ -320,6 +328,16 @@ void eri_run(Context &C, const EriRequest &R) {
       }
       nq = nrows;
     }
+    // lane-per-quartet kernel for small uncontracted classes
+    bool lane_path = (L.E <= LANE_EMAX && L.F <= LANE_EMAX);
+    if (lane_path) {
+      for (int r : L.rows)
+        if (C.shell_np[rd[r].s0] * C.shell_np[rd[r].s1] != 1) { lane_path = false; break; }
+    }
+    if (lane_path) {
+      for (int c : L.cols)
+        if (C.shell_np[cd[c].s0] * C.shell_np[cd[c].s1] != 1) { lane_path = false; break; }
+    }
     MdArgs a{};
     a.hdr = C.md_hdr.p;
     a.prim = C.md_prim.p;
@@ -344,15 +362,37 @@ void eri_run(Context &C, const EriRequest &R) {
     a.ldo = R.ldo;
     a.sel = R.d_sel;
     a.boys = C.dboys.p;
-    long long grid = std::min<long long>(a.ntiles, 4096);
-    if (grid >= 8) grid = (grid + 7) / 8 * 8;
+    long long grid;
+    if (lane_path) {
+      const long long ng = (nrows + 63) / 64;
+      const long long nt = (R.mode == LAUNCH_PRODUCT) ? ng * ncols : ng;
+      grid = std::min<long long>((nt + 3) / 4, 16384);
+      // the lane kernel's own operation count: 2 nr NHF (NHE + nc) per quartet
+      fl = 0;
+      if (R.mode == LAUNCH_PRODUCT) {
+        double sr = 0, sc = 0;
+        for (int r : L.rows) sr += pair_rows_host(rd[r], C.shell_nang[rd[r].s0], C.shell_nang[rd[r].s1]);
+        for (int c : L.cols) sc += pair_rows_host(cd[c], C.shell_nang[cd[c].s0], C.shell_nang[cd[c].s1]);
+        fl = 2.0 * NHF * (NHE * sr * (double)ncols + sr * sc) + rt * (double)nrows * (double)ncols;
+      } else {
+        for (size_t k = 0; k < L.rows.size(); ++k) {
+          const PairDesc &x = rd[L.rows[k]], &y = cd[L.cols[k]];
+          const double nr = pair_rows_host(x, C.shell_nang[x.s0], C.shell_nang[x.s1]);
+          const double nc = (R.mode == LAUNCH_DIAG) ? 1.0 : pair_rows_host(y, C.shell_nang[y.s0], C.shell_nang[y.s1]);
+          fl += 2.0 * nr * NHF * (NHE + nc) + rt;
+        }
+      }
+    } else {
+      grid = std::min<long long>(a.ntiles, 4096);
+      if (grid >= 8) grid = (grid + 7) / 8 * 8;
+    }
     hipEvent_t s0 = nullptr, s1 = nullptr;
     if (stats) {
       GI_HIP(hipEventCreate(&s0));
       GI_HIP(hipEventCreate(&s1));
       GI_HIP(hipEventRecord(s0, C.stream));
     }
-    md_table[L.E][L.F](a, (int)grid, C.stream);
+    (lane_path ? lane_table[L.E][L.F] : md_table[L.E][L.F])(a, (int)grid, C.stream);
     GI_HIP(hipGetLastError());
     if (stats) {
       GI_HIP(hipEventRecord(s1, C.stream));

@@ -371,4 +371,145 @@ __global__ void __launch_bounds__(64) md_kernel(MdArgs A) {
   }
 }
 
+// ------------------------------------------------------------- lane kernel
+// Small classes (la+lb <= 2, lc+ld <= 2, uncontracted pairs): one lane per
+// quartet, everything in registers.  A wave takes 64 consecutive bra pairs of
+// the row list against one ket pair (PRODUCT; the ket side is then uniform
+// across the wave and its loads scalar), or 64 consecutive zipped pairs.
+//   y(k)    = sum_h ET_bra(h, r) R_{h+k}
+//   V(r, c) = sum_k (-1)^{|k|} y(k) ET_ket(k, c)
+constexpr int LANE_EMAX = 2;
+
+__host__ __device__ constexpr int hsum(int h, int k) {
+  return hidx(px(herm_of(h)) + px(herm_of(k)), py(herm_of(h)) + py(herm_of(k)), pz(herm_of(h)) + pz(herm_of(k)));
+}
+__host__ __device__ constexpr int hord(int h) { return px(herm_of(h)) + py(herm_of(h)) + pz(herm_of(h)); }
+
+// compile-time index tables (so that unrolled register code has constant indices)
+template <int E, int F>
+struct SumTab {
+  int v[nherm(E)][nherm(F)];
+  int odd[nherm(F)];
+  constexpr SumTab() : v(), odd() {
+    for (int h = 0; h < nherm(E); ++h)
+      for (int k = 0; k < nherm(F); ++k) v[h][k] = hsum(h, k);
+    for (int k = 0; k < nherm(F); ++k) odd[k] = hord(k) & 1;
+  }
+};
+template <int L>
+struct RecTab {
+  int i1[nherm(L)], i2[nherm(L)], dir[nherm(L)], c[nherm(L)];
+  constexpr RecTab() : i1(), i2(), dir(), c() {
+    for (int idx = 1; idx < nherm(L); ++idx) {
+      const int p = herm_of(idx);
+      const int t = px(p), u = py(p), v = pz(p);
+      const int d = t > 0 ? 0 : (u > 0 ? 1 : 2);
+      dir[idx] = d;
+      c[idx] = d == 0 ? t : (d == 1 ? u : v);
+      i1[idx] = hidx(t - (d == 0), u - (d == 1), v - (d == 2));
+      i2[idx] = c[idx] >= 2 ? hidx(t - 2 * (d == 0), u - 2 * (d == 1), v - 2 * (d == 2)) : 0;
+    }
+  }
+};
+
+// R_{tuv} of one primitive quartet in registers, recursion in place: level n
+// entries in decreasing index order only read lower indices of level n + 1.
+template <int L>
+__device__ __forceinline__ void rtable_reg(double (&R)[nherm(L)], const PrimRec &pb, const PrimRec &pk,
+                                           double kappa, const double *__restrict__ boys) {
+  const double PQ[3] = {pb.px - pk.px, pb.py - pk.py, pb.pz - pk.pz};
+  const double pq = pb.p * pk.p;
+  const double den = pb.p + pk.p + kappa * pq;
+  const double alpha = pq / den;
+  const double N = 34.986836655249725 / (pq * sqrt(den));   // 2 pi^(5/2)
+  double Fm[L + 1];
+  boys_fm<L>(alpha * (PQ[0] * PQ[0] + PQ[1] * PQ[1] + PQ[2] * PQ[2]), boys, Fm);
+  const double m2a = -2.0 * alpha;
+  double pw = N;
+#pragma unroll
+  for (int n = 0; n <= L; ++n) {
+    Fm[n] *= pw;
+    pw *= m2a;
+  }
+  constexpr RecTab<L> T{};
+  R[0] = Fm[L];
+#pragma unroll
+  for (int n = L - 1; n >= 0; --n) {
+#pragma unroll
+    for (int idx = nherm(L - n) - 1; idx >= 1; --idx) {
+      double r = PQ[T.dir[idx]] * R[T.i1[idx]];
+      if (T.c[idx] >= 2) r += (double)(T.c[idx] - 1) * R[T.i2[idx]];
+      R[idx] = r;
+    }
+    R[0] = Fm[n];
+  }
+}
+
+template <int E, int F>
+__global__ void __launch_bounds__(256) md_lane_kernel(MdArgs A) {
+  constexpr int NHE = nherm(E), NHF = nherm(F), L = E + F;
+  constexpr int NRM = nab_max(E), NCM = nab_max(F);
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const long long nrg = (A.mode == LAUNCH_PRODUCT) ? (A.nrows + 63) / 64 : (A.nrows + 63) / 64;
+  const long long ntile = (A.mode == LAUNCH_PRODUCT) ? nrg * A.ncols : nrg;
+  for (long long t = (long long)blockIdx.x * 4 + wave; t < ntile; t += (long long)gridDim.x * 4) {
+    long long ir, jc;
+    if (A.mode == LAUNCH_PRODUCT) {
+      const long long rg = t / A.ncols;
+      jc = t - rg * A.ncols;
+      ir = rg * 64 + lane;
+    } else {
+      ir = t * 64 + lane;
+      jc = ir;
+    }
+    if (ir >= A.nrows) continue;
+    const PairDesc RD = A.rdesc[A.rows[ir]];
+    const PairDesc CD = A.cdesc[A.cols[jc]];
+    const PairHdr BH = A.hdr[RD.pt], KH = A.hdr[CD.pt];
+    const PrimRec PB = A.prim[BH.prim0], PK = A.prim[KH.prim0];
+    const int nr = pair_nrow(RD, BH), nc = pair_nrow(CD, KH);
+    const int ldb = BH.n0 * BH.n1, ldk = KH.n0 * KH.n1;
+    const double *etb = A.et + PB.et, *etk = A.et + PK.et;
+    double *dst = A.out + RD.off + CD.off;
+    double R[nherm(L)];
+    rtable_reg<L>(R, PB, PK, A.kappa, A.boys);
+#pragma unroll 1
+    for (int r = 0; r < NRM; ++r) {
+      if (r < nr) {
+        const int fr = pair_full(RD, BH, A.sel, r);
+        double eb[NHE];
+#pragma unroll
+        for (int h = 0; h < NHE; ++h) eb[h] = etb[(long long)h * ldb + fr];
+        constexpr SumTab<E, F> ST{};
+        double y[NHF];
+#pragma unroll
+        for (int k = 0; k < NHF; ++k) {
+          double s = 0.0;
+#pragma unroll
+          for (int h = 0; h < NHE; ++h) s += eb[h] * R[ST.v[h][k]];
+          y[k] = ST.odd[k] ? -s : s;
+        }
+        if (A.mode == LAUNCH_DIAG) {
+          // diagonal element (r, r) only
+          double v = 0.0;
+#pragma unroll
+          for (int k = 0; k < NHF; ++k) v += y[k] * etk[(long long)k * ldk + fr];
+          dst[r] = v;
+        } else {
+#pragma unroll
+          for (int c = 0; c < NCM; ++c) {
+            if (c < nc) {
+              const int fc = pair_full(CD, KH, A.sel, c);
+              double v = 0.0;
+#pragma unroll
+              for (int k = 0; k < NHF; ++k) v += y[k] * etk[(long long)k * ldk + fc];
+              dst[r + A.ldo * c] = v;
+            }
+          }
+        }
+      }
+    }
+  }
+}
+
 }  // namespace gi
