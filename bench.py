@@ -87,6 +87,9 @@ def main():
     ap.add_argument("--no-mo", action="store_true")
     ap.add_argument("--eri-stats", action="store_true",
                     help="print per-class ERI kernel statistics to stderr (sets GIGPU_ERI_STATS)")
+    ap.add_argument("--copy-probe-mb", type=int, default=0,
+                    help="before the run, one device-to-device copy of this many MiB (a dispatch of known "
+                         "size for scaling hardware byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (see DESIGN.md, sharding)",
@@ -105,6 +108,14 @@ def main():
     Cmo = np.asfortranarray(Cmo)
     ctx = lib.Context(0)
     hip = Hip()
+    if args.copy_probe_mb > 0:
+        nbytes = args.copy_probe_mb << 20
+        src, dst = hip.alloc(nbytes), hip.alloc(nbytes)
+        if hip.L.hipMemcpy(C.c_void_p(dst), C.c_void_p(src), nbytes, 3) != 0:
+            raise RuntimeError("hipMemcpy failed")
+        hip.sync()
+        hip.free(src)
+        hip.free(dst)
     dC = hip.alloc(Cmo.nbytes)
     hip.h2d(dC, Cmo)
     dR = None
@@ -143,6 +154,11 @@ def main():
     eri_s = tm["eri_kernel"]
     eri_flops = tm["eri_flops"]
     achieved = eri_flops / eri_s / 1e12 if eri_s > 0 else None
+    # Rkpq = Inv_L Vabrs^T with Inv_L lower triangular, one GEMM per block row
+    nv = d["nvecs"]
+    nblk = max(1, min(8, nv // 512))
+    gemm_flops = sum(2.0 * (nv * (rb + 1) // nblk - nv * rb // nblk) * (nv * (rb + 1) // nblk)
+                     for rb in range(nblk)) * d["norb_pairs"]
     out = {
         "metric": METRIC,
         "value": quartets / t_build,
@@ -160,6 +176,9 @@ def main():
                                "+ AO->MO transform nocc=40 nvirt=300" % args.nwater,
                    "nao": basis.nao, "tau": 1e-6, "form": "plain FP64 GPU (batched/mixed forms not built)"},
         "cholesky_build_s": t_build,
+        "dims": {k: d[k] for k in ("nvecs", "norb_pairs", "nshell_pairs", "nsubsets", "max_subset_dim",
+                                   "npivot_shell_pairs", "nmacro")},
+        "vectors_gemm_tflops": gemm_flops / tm["vectors_gemm"] / 1e12 if tm["vectors_gemm"] > 0 else None,
         "mo_transform_s": t_mo,
         "nvecs": d["nvecs"],
         "eri_quartets": quartets,

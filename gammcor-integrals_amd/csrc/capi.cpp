@@ -166,6 +166,54 @@ int gi_eri_quartets(gi_context *cp, int64_t nq, const int64_t *quartets, double 
   })
 }
 
+int gi_eri_pair_block(gi_context *cp, int64_t nrp, const int64_t *row_pairs, int64_t ncp, const int64_t *col_pairs,
+                      double kappa, double *out, int64_t ldo) {
+  GI_TRY({
+    gi::Context &C = ctx_of(cp);
+    if (!C.basis_set) throw Error{4, "basis not set"};
+    if (nrp <= 0 || ncp <= 0) throw Error{8, "empty pair list"};
+    std::vector<gi::PairDesc> rd, cd;
+    std::vector<int> rows, cols;
+    long long nr = 0, nc = 0;
+    for (int64_t i = 0; i < nrp; ++i) {
+      const int64_t a = row_pairs[2 * i], b = row_pairs[2 * i + 1];
+      if (a < 0 || a >= C.nshells || b < 0 || b >= C.nshells) throw Error{8, "shell index out of range"};
+      rd.push_back(gi::PairDesc{(int)a, (int)b, 0, 0, 0, 0, nr});
+      rows.push_back((int)i);
+      nr += (long long)C.shell_nang[a] * C.shell_nang[b];
+    }
+    if (ldo < nr) throw Error{8, "leading dimension smaller than the number of rows"};
+    for (int64_t i = 0; i < ncp; ++i) {
+      const int64_t a = col_pairs[2 * i], b = col_pairs[2 * i + 1];
+      if (a < 0 || a >= C.nshells || b < 0 || b >= C.nshells) throw Error{8, "shell index out of range"};
+      cd.push_back(gi::PairDesc{(int)a, (int)b, 0, 0, 0, 0, ldo * nc});
+      cols.push_back((int)i);
+      nc += (long long)C.shell_nang[a] * C.shell_nang[b];
+    }
+    gi::DBuf<double> dout;
+    dout.alloc((size_t)ldo * nc);
+    GI_HIP(hipMemsetAsync(dout.p, 0, sizeof(double) * ldo * nc, C.stream));
+    gi::EriRequest R;
+    R.rdesc = &rd; R.cdesc = &cd;
+    R.rows = &rows; R.cols = &cols; R.mode = gi::LAUNCH_PRODUCT;
+    R.out = dout.p; R.ldo = ldo; R.kappa = kappa;
+    gi::eri_run(C, R);
+    GI_HIP(hipMemcpy(out, dout.p, sizeof(double) * ldo * nc, hipMemcpyDeviceToHost));
+  })
+}
+
+int gi_set_option(gi_context *cp, const char *name, int64_t value) {
+  GI_TRY({
+    gi::Context &C = ctx_of(cp);
+    const std::string n = name ? name : "";
+    if (n == "eri_min_tiles") C.eri_min_tiles = std::max<int64_t>(1, value);
+    else if (n == "eri_max_kt") C.eri_max_kt = std::max<int64_t>(1, value);
+    else if (n == "eri_lane") C.eri_lane = value != 0;
+    else if (n == "eri_trans") C.eri_trans = value != 0;
+    else throw Error{9, "unknown option " + n};
+  })
+}
+
 int gi_chol_accuracy_tau(int64_t accuracy, double *tau) {
   GI_TRY({
     switch (accuracy) {

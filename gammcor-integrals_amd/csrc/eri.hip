@@ -5,6 +5,7 @@
 // pairing) of every bra bin with every ket bin.  One instantiation per
 // (E, F), listed by gigpu/eri_gen.py in md_classes.inc.
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <map>
 #include <unordered_map>
@@ -288,75 +289,77 @@ void eri_run(Context &C, const EriRequest &R) {
   static const bool stats = getenv("GIGPU_ERI_STATS") != nullptr;
   long long nq_total = 0;
   double flops_total = 0;
+  auto nrow_of = [&](const PairDesc &d) {
+    return (double)pair_rows_host(d, C.shell_nang[d.s0], C.shell_nang[d.s1]);
+  };
+  auto nprim_of = [&](const PairDesc &d) { return (double)C.shell_np[d.s0] * C.shell_np[d.s1]; };
   for (size_t i = 0; i < launches.size(); ++i) {
     auto &L = launches[i];
-    const long long nrows = (long long)L.rows.size(), ncols = (long long)L.cols.size();
+    long long nrows = (long long)L.rows.size(), ncols = (long long)L.cols.size();
     if (nrows == 0 || ncols == 0) continue;
-    // algorithmic operation count of the launch (useful rows/columns only)
-    const double NHE = nherm(L.E), NHF = nherm(L.F), rt = rtable_flops(L.E + L.F);
+    const long long nq = (R.mode == LAUNCH_PRODUCT) ? nrows * ncols : nrows;
+    // lane-per-quartet kernel for small uncontracted classes
+    bool lane_path = C.eri_lane && (L.E <= LANE_EMAX && L.F <= LANE_EMAX);
+    for (size_t k = 0; lane_path && k < L.rows.size(); ++k) lane_path = nprim_of(rd[L.rows[k]]) == 1;
+    for (size_t k = 0; lane_path && k < L.cols.size(); ++k) lane_path = nprim_of(cd[L.cols[k]]) == 1;
+    // MFMA kernel orientation: its bra side (stage-2 rows, 16-row tiles) is
+    // the side with more rows per pair; the thin side is stacked in columns
+    bool trans = false;
+    if (!lane_path && R.mode == LAUNCH_PRODUCT && C.eri_trans) {
+      double sr = 0, sc = 0;
+      for (int r : L.rows) sr += nrow_of(rd[r]);
+      for (int c : L.cols) sc += nrow_of(cd[c]);
+      trans = sc / (double)ncols > sr / (double)nrows;
+    }
+    const int KE = trans ? L.F : L.E, KF = trans ? L.E : L.F;   // kernel bra / ket orders
+    const std::vector<PairDesc> &krd = trans ? cd : rd, &kcd = trans ? rd : cd;
+    const std::vector<int> &krows = trans ? L.cols : L.rows, &kcols = trans ? L.rows : L.cols;
+    if (trans) std::swap(nrows, ncols);
+    // algorithmic operation count in the kernel's own roles (useful rows/columns only)
+    const double NHE = nherm(KE), NHF = nherm(KF), rt = rtable_flops(L.E + L.F);
     double fl = 0;
-    long long nq = 0;
-    auto info = [&](const PairDesc &d, double &nrow, double &np) {
-      nrow = pair_rows_host(d, C.shell_nang[d.s0], C.shell_nang[d.s1]);
-      np = (double)C.shell_np[d.s0] * C.shell_np[d.s1];
-    };
     if (R.mode == LAUNCH_PRODUCT) {
-      double sr_np = 0, sr_nrnp = 0, sc_np = 0, sc_ncnp = 0, sc_nc = 0;
-      for (int r : L.rows) {
-        double n, p;
-        info(rd[r], n, p);
+      double sr_np = 0, sr_nr = 0, sr_nrnp = 0, sc_np = 0, sc_nc = 0, sc_ncnp = 0;
+      for (int r : krows) {
+        const double n = nrow_of(krd[r]), p = nprim_of(krd[r]);
         sr_np += p;
+        sr_nr += n;
         sr_nrnp += n * p;
       }
-      for (int c : L.cols) {
-        double n, p;
-        info(cd[c], n, p);
+      for (int c : kcols) {
+        const double n = nrow_of(kcd[c]), p = nprim_of(kcd[c]);
         sc_np += p;
-        sc_ncnp += n * p;
         sc_nc += n;
+        sc_ncnp += n * p;
       }
-      // stage 1: 2 NHE NHF nc per primitive quartet; stage 2: 2 nr NHE nc per bra primitive pair
-      fl = 2.0 * NHE * NHF * sr_np * sc_ncnp + 2.0 * NHE * sr_nrnp * sc_nc + rt * sr_np * sc_np;
-      nq = nrows * ncols;
+      if (lane_path)   // 2 nr NHF (NHE + nc) per quartet
+        fl = 2.0 * NHF * (NHE * sr_nr * (double)ncols + sr_nr * sc_nc) + rt * (double)nrows * (double)ncols;
+      else             // stage 1 2 NHE NHF nc per primitive quartet, stage 2 2 nr NHE nc per bra primitive pair
+        fl = 2.0 * NHE * NHF * sr_np * sc_ncnp + 2.0 * NHE * sr_nrnp * sc_nc + rt * sr_np * sc_np;
     } else {
-      for (size_t k = 0; k < L.rows.size(); ++k) {
-        double nr, pr, nc, pc;
-        info(rd[L.rows[k]], nr, pr);
-        info(cd[L.cols[k]], nc, pc);
-        const double st2 = (R.mode == LAUNCH_DIAG) ? 2.0 * NHE * nc * pr : 2.0 * NHE * nr * nc * pr;
-        fl += 2.0 * NHE * NHF * nc * pr * pc + st2 + rt * pr * pc;
+      for (size_t k = 0; k < krows.size(); ++k) {
+        const double nr = nrow_of(krd[krows[k]]), pr = nprim_of(krd[krows[k]]);
+        const double nc = nrow_of(kcd[kcols[k]]), pc = nprim_of(kcd[kcols[k]]);
+        if (lane_path)
+          fl += 2.0 * nr * NHF * (NHE + (R.mode == LAUNCH_DIAG ? 1.0 : nc)) + rt;
+        else
+          fl += 2.0 * NHE * NHF * nc * pr * pc + (R.mode == LAUNCH_DIAG ? 2.0 * NHE * nc * pr : 2.0 * NHE * nr * nc * pr) +
+                rt * pr * pc;
       }
-      nq = nrows;
-    }
-    // lane-per-quartet kernel for small uncontracted classes
-    bool lane_path = (L.E <= LANE_EMAX && L.F <= LANE_EMAX);
-    if (lane_path) {
-      for (int r : L.rows)
-        if (C.shell_np[rd[r].s0] * C.shell_np[rd[r].s1] != 1) { lane_path = false; break; }
-    }
-    if (lane_path) {
-      for (int c : L.cols)
-        if (C.shell_np[cd[c].s0] * C.shell_np[cd[c].s1] != 1) { lane_path = false; break; }
     }
     MdArgs a{};
     a.hdr = C.md_hdr.p;
     a.prim = C.md_prim.p;
     a.et = C.md_et.p;
-    a.rdesc = C.md_rdesc.p;
-    a.cdesc = C.md_cdesc.p;
-    a.rows = C.md_idx.p + roff[i];
-    a.cols = C.md_idx.p + coff[i];
+    a.rdesc = trans ? C.md_cdesc.p : C.md_rdesc.p;
+    a.cdesc = trans ? C.md_rdesc.p : C.md_cdesc.p;
+    a.rows = C.md_idx.p + (trans ? coff[i] : roff[i]);
+    a.cols = C.md_idx.p + (trans ? roff[i] : coff[i]);
     a.nrows = nrows;
     a.ncols = ncols;
     a.mode = R.mode;
+    a.trans = trans ? 1 : 0;
     a.kt = 1;
-    if (R.mode == LAUNCH_PRODUCT) {
-      const long long q = nrows * ncols;
-      a.kt = (int)std::max<long long>(1, std::min<long long>(4, q / 65536));
-      a.ntiles = nrows * ((ncols + a.kt - 1) / a.kt);
-    } else {
-      a.ntiles = nrows;
-    }
     a.kappa = R.kappa;
     a.out = R.out;
     a.ldo = R.ldo;
@@ -366,23 +369,22 @@ void eri_run(Context &C, const EriRequest &R) {
     if (lane_path) {
       const long long ng = (nrows + 63) / 64;
       const long long nt = (R.mode == LAUNCH_PRODUCT) ? ng * ncols : ng;
+      a.ntiles = nt;
       grid = std::min<long long>((nt + 3) / 4, 16384);
-      // the lane kernel's own operation count: 2 nr NHF (NHE + nc) per quartet
-      fl = 0;
-      if (R.mode == LAUNCH_PRODUCT) {
-        double sr = 0, sc = 0;
-        for (int r : L.rows) sr += pair_rows_host(rd[r], C.shell_nang[rd[r].s0], C.shell_nang[rd[r].s1]);
-        for (int c : L.cols) sc += pair_rows_host(cd[c], C.shell_nang[cd[c].s0], C.shell_nang[cd[c].s1]);
-        fl = 2.0 * NHF * (NHE * sr * (double)ncols + sr * sc) + rt * (double)nrows * (double)ncols;
-      } else {
-        for (size_t k = 0; k < L.rows.size(); ++k) {
-          const PairDesc &x = rd[L.rows[k]], &y = cd[L.cols[k]];
-          const double nr = pair_rows_host(x, C.shell_nang[x.s0], C.shell_nang[x.s1]);
-          const double nc = (R.mode == LAUNCH_DIAG) ? 1.0 : pair_rows_host(y, C.shell_nang[y.s0], C.shell_nang[y.s1]);
-          fl += 2.0 * nr * NHF * (NHE + nc) + rt;
-        }
-      }
     } else {
+      if (R.mode == LAUNCH_PRODUCT) {
+        // enough ket pairs per tile to fill 32 stacked columns, while keeping
+        // at least eri_min_tiles tiles when the launch is large enough
+        double sc = 0;
+        for (int c : kcols) sc += nrow_of(kcd[c]);
+        long long kt = (long long)std::ceil(32.0 / std::max(1.0, sc / (double)ncols));
+        kt = std::max<long long>(1, std::min<long long>(kt, C.eri_max_kt));
+        while (kt > 1 && nrows * ((ncols + kt - 1) / kt) < C.eri_min_tiles) kt = (kt + 1) / 2;
+        a.kt = (int)kt;
+        a.ntiles = nrows * ((ncols + a.kt - 1) / a.kt);
+      } else {
+        a.ntiles = nrows;
+      }
       grid = std::min<long long>(a.ntiles, 4096);
       if (grid >= 8) grid = (grid + 7) / 8 * 8;
     }
@@ -392,7 +394,7 @@ void eri_run(Context &C, const EriRequest &R) {
       GI_HIP(hipEventCreate(&s1));
       GI_HIP(hipEventRecord(s0, C.stream));
     }
-    (lane_path ? lane_table[L.E][L.F] : md_table[L.E][L.F])(a, (int)grid, C.stream);
+    (lane_path ? lane_table[KE][KF] : md_table[KE][KF])(a, (int)grid, C.stream);
     GI_HIP(hipGetLastError());
     if (stats) {
       GI_HIP(hipEventRecord(s1, C.stream));

@@ -88,6 +88,9 @@ struct Context {
   // per kernel class (key 16*E + F, E = la+lb, F = lc+ld): quartets, kernel ms, FP64 ops
   // (filled when GIGPU_ERI_STATS is set in the environment)
   std::map<int, std::array<double, 3>> class_stats;
+  // launch-layer tuning (gi_set_option)
+  long long eri_min_tiles = 8192, eri_max_kt = 32;
+  bool eri_lane = true, eri_trans = true;
 
   Context(int dev);
   ~Context();

@@ -140,6 +140,7 @@ struct MdArgs {
   const int *rows, *cols;        // indices into rdesc / cdesc
   long long nrows, ncols, ntiles;
   int mode, kt;                  // LAUNCH_*, ket pairs per tile (PRODUCT)
+  int trans;                     // kernel bra side = caller's ket side (store transposed)
   double kappa;
   double *out;
   long long ldo;
@@ -215,14 +216,32 @@ __device__ __forceinline__ void rtable(double *Ra, double *Rb, const PrimRec &pb
 }
 
 // ------------------------------------------------------------- quartet kernel
+// One wavefront per tile of one bra pair and A.kt ket pairs.  Stage-1 results
+// X(h, c) of successive ket columns (16 at a time) are appended to a ring of
+// 32 columns in LDS; every 16 queued columns -- possibly from different ket
+// pairs, which share ET_bra -- go through stage 2 together, so thin ket pairs
+// (ss, ps, a few selected pivots) still fill the 16-wide MFMA tiles.  A
+// contracted bra pair runs the whole tile once per bra primitive pair and
+// accumulates into the destination.  In a transposed launch (A.trans) the
+// kernel's bra side is the caller's ket side and rows/columns of the stored
+// block are exchanged.
+constexpr int MD_RING = 32;
+constexpr int MD_NCV = 4;   // slices of at most this many columns take stage 1 on the VALU
+
 template <int E, int F>
 __global__ void __launch_bounds__(64) md_kernel(MdArgs A) {
   using S = MdShape<E, F>;
   constexpr int HT = S::HT, KSF = S::KSF, KSE = S::KSE, RT = S::RT, NHE = S::NHE, NHF = S::NHF;
   constexpr int NHL = S::NHL;
+  // unroll factors of the k-step loops: full for small classes, partial where
+  // hoisting every operand load of a fully unrolled loop would spill
+  constexpr int U1 = KSF * HT > 24 ? 1 : KSF;
+  constexpr int U2 = KSE * RT > 24 ? 1 : KSE;
   __shared__ unsigned short s_idx[S::NIDX];
   __shared__ double s_R[2][NHL + 1];
-  __shared__ double s_X[HT * 16 * 16];
+  __shared__ double s_X[HT * 16][MD_RING];
+  __shared__ long long s_qbase[MD_RING];
+  __shared__ double s_E[NHF][MD_NCV];
 
   const int lane = threadIdx.x;
   const int i16 = lane & 15, g = lane >> 4;
@@ -241,6 +260,7 @@ __global__ void __launch_bounds__(64) md_kernel(MdArgs A) {
     }
     s_idx[e] = v;
   }
+  for (int e = lane; e < HT * 16 * MD_RING; e += 64) (&s_X[0][0])[e] = 0.0;
   if (lane == 0) {
     s_R[0][NHL] = 0.0;
     s_R[1][NHL] = 0.0;
@@ -252,6 +272,7 @@ __global__ void __launch_bounds__(64) md_kernel(MdArgs A) {
   const int w = blockIdx.x;
   const long long v0 = (G % 8 == 0) ? (long long)(w % 8) * (G / 8) + w / 8 : w;
   const long long nct = (A.mode == LAUNCH_PRODUCT) ? (A.ncols + A.kt - 1) / A.kt : 1;
+  const long long rs = A.trans ? A.ldo : 1;   // destination stride of a kernel row
 
   for (long long t = v0; t < A.ntiles; t += G) {
     long long ir, jc0;
@@ -276,26 +297,116 @@ __global__ void __launch_bounds__(64) md_kernel(MdArgs A) {
       fr[rt] = pair_full(RD, BH, A.sel, r < nr ? r : nr - 1);
     }
     const int nrt = (nr + 15) / 16;
+    double *const out = A.out + RD.off;
 
-    for (int kq = 0; kq < nk; ++kq) {
-      const PairDesc CD = A.cdesc[A.cols[jc0 + kq]];
-      const PairHdr KH = A.hdr[CD.pt];
-      const int nc = pair_nrow(CD, KH);
-      const int ldk = KH.n0 * KH.n1;
-      double *dst = A.out + RD.off + CD.off;
-      const bool single = (BH.nprim == 1 && KH.nprim == 1);
-      if (single) rtable<E + F>(s_R[0], s_R[1], A.prim[BH.prim0], A.prim[KH.prim0], A.kappa, A.boys, lane);
+    for (int bp = 0; bp < BH.nprim; ++bp) {
+      const PrimRec PB = A.prim[BH.prim0 + bp];
+      const double *etb = A.et + PB.et;
+      const bool accumulate = bp > 0;
+      int head = 0, q = 0;   // ring of queued stage-1 columns
 
-      for (int c0 = 0; c0 < nc; c0 += 16) {
-        const int cc = c0 + i16;
-        const int fc = pair_full(CD, KH, A.sel, cc < nc ? cc : nc - 1);
+      // stage 2 on the n oldest queued columns, stored (or added) to out
+      auto flush = [&](int n) {
         d4 vacc[RT];
 #pragma unroll
         for (int rt = 0; rt < RT; ++rt) vacc[rt] = d4{0.0, 0.0, 0.0, 0.0};
-        double dacc = 0.0;
+        const int col = (head + i16) & (MD_RING - 1);
+#pragma unroll U2
+        for (int ks = 0; ks < KSE; ++ks) {
+          const int h = 4 * ks + g;
+          const double a = s_X[h][col];
+#pragma unroll
+          for (int rt = 0; rt < RT; ++rt) {
+            if (rt < nrt) {
+              const double b = (h < NHE) ? etb[(long long)h * ldb + fr[rt]] : 0.0;
+              vacc[rt] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, vacc[rt], 0, 0, 0);
+            }
+          }
+        }
+        // lane (j, g) holds V(row 16 rt + j, queued column 4 qq + g)
+#pragma unroll
+        for (int qq = 0; qq < 4; ++qq) {
+          const int ci = 4 * qq + g;
+          if (ci < n) {
+            const long long base = s_qbase[(head + ci) & (MD_RING - 1)];
+#pragma unroll
+            for (int rt = 0; rt < RT; ++rt) {
+              const int r = 16 * rt + i16;
+              if (rt < nrt && r < nr) {
+                double *d = out + base + rs * r;
+                *d = accumulate ? *d + vacc[rt][qq] : vacc[rt][qq];
+              }
+            }
+          }
+        }
+        head = (head + n) & (MD_RING - 1);
+        q -= n;
+        __syncthreads();   // ring slots are free again
+      };
 
-        for (int bp = 0; bp < BH.nprim; ++bp) {
-          const PrimRec PB = A.prim[BH.prim0 + bp];
+      for (int kq = 0; kq < nk; ++kq) {
+        const PairDesc CD = A.cdesc[A.cols[jc0 + kq]];
+        const PairHdr KH = A.hdr[CD.pt];
+        const int nc = pair_nrow(CD, KH);
+        const int ldk = KH.n0 * KH.n1;
+        const bool single = (KH.nprim == 1);
+        if (single) rtable<E + F>(s_R[0], s_R[1], PB, A.prim[KH.prim0], A.kappa, A.boys, lane);
+
+        for (int c0 = 0; c0 < nc; c0 += 16) {
+          const int cc = c0 + i16;
+          const int ncs = (nc - c0) < 16 ? (nc - c0) : 16;
+          if (ncs <= MD_NCV && A.mode != LAUNCH_DIAG) {
+            // narrow slice: X(h, c) = sum_k (-1)^{|k|} R_{h+k} ET_ket(k, c) on the
+            // VALU, lane h (and h + 64), the slice of ET_ket staged in LDS
+            double xv[2][MD_NCV];
+#pragma unroll
+            for (int ps = 0; ps < 2; ++ps)
+#pragma unroll
+              for (int c = 0; c < MD_NCV; ++c) xv[ps][c] = 0.0;
+            for (int kp = 0; kp < KH.nprim; ++kp) {
+              const PrimRec PK = A.prim[KH.prim0 + kp];
+              if (!single) rtable<E + F>(s_R[0], s_R[1], PB, PK, A.kappa, A.boys, lane);
+              const double *etk = A.et + PK.et;
+              for (int e = lane; e < NHF * MD_NCV; e += 64) {
+                const int k = e / MD_NCV, c = e - k * MD_NCV;
+                s_E[k][c] = (c < ncs) ? etk[(long long)k * ldk + pair_full(CD, KH, A.sel, c0 + c)] : 0.0;
+              }
+              __syncthreads();
+              const double *Rf = s_R[0];
+#pragma unroll
+              for (int ps = 0; ps < 2; ++ps) {
+                const int h = lane + 64 * ps;
+                if (h < NHE) {
+#pragma unroll 4
+                  for (int k = 0; k < NHF; ++k) {
+                    const unsigned short e = s_idx[((k >> 2) * HT + (h >> 4)) * 64 + (h & 15) + 16 * (k & 3)];
+                    double r = Rf[e & 0x7fff];
+                    r = (e & 0x8000) ? -r : r;
+#pragma unroll
+                    for (int c = 0; c < MD_NCV; ++c) xv[ps][c] += r * s_E[k][c];
+                  }
+                }
+              }
+              __syncthreads();
+            }
+#pragma unroll
+            for (int c = 0; c < MD_NCV; ++c) {
+              if (c < ncs) {
+                const int slot = (head + q + c) & (MD_RING - 1);
+#pragma unroll
+                for (int ps = 0; ps < 2; ++ps) {
+                  const int h = lane + 64 * ps;
+                  if (h < HT * 16) s_X[h][slot] = h < NHE ? xv[ps][c] : 0.0;
+                }
+                if (lane == 0) s_qbase[slot] = A.trans ? CD.off + (c0 + c) : CD.off + A.ldo * (c0 + c);
+              }
+            }
+            q += ncs;
+            __syncthreads();
+            if (q >= 16) flush(16);
+            continue;
+          }
+          const int fc = pair_full(CD, KH, A.sel, cc < nc ? cc : nc - 1);
           d4 xacc[HT];
 #pragma unroll
           for (int ht = 0; ht < HT; ++ht) xacc[ht] = d4{0.0, 0.0, 0.0, 0.0};
@@ -304,7 +415,7 @@ __global__ void __launch_bounds__(64) md_kernel(MdArgs A) {
             if (!single) rtable<E + F>(s_R[0], s_R[1], PB, PK, A.kappa, A.boys, lane);
             const double *etk = A.et + PK.et;
             const double *Rf = s_R[0];
-#pragma unroll
+#pragma unroll U1
             for (int ks = 0; ks < KSF; ++ks) {
               const int k = 4 * ks + g;
               const double b = (k < NHF) ? etk[(long long)k * ldk + fc] : 0.0;
@@ -317,56 +428,41 @@ __global__ void __launch_bounds__(64) md_kernel(MdArgs A) {
               }
             }
           }
-          // X(h, c) to LDS.  MFMA f64 16x16x4 result layout: element (i, j) of a
-          // tile sits in lane j + 16 (i % 4), register i / 4, so lane (j, g)
-          // holds rows 16 ht + 4 q + g of column j.
-          __syncthreads();
-#pragma unroll
-          for (int ht = 0; ht < HT; ++ht)
-#pragma unroll
-            for (int q = 0; q < 4; ++q) s_X[(16 * ht + 4 * q + g) * 16 + i16] = xacc[ht][q];
-          __syncthreads();
-          const double *etb = A.et + PB.et;
           if (A.mode == LAUNCH_DIAG) {
             // diagonal elements only: V(c, c) = sum_h ET_bra(h, c) X(h, c)
-            if (g == 0) {
+            __syncthreads();
+#pragma unroll
+            for (int ht = 0; ht < HT; ++ht)
+#pragma unroll
+              for (int qq = 0; qq < 4; ++qq) s_X[16 * ht + 4 * qq + g][i16] = xacc[ht][qq];
+            __syncthreads();
+            if (g == 0 && cc < nc) {
               double s = 0.0;
 #pragma unroll 4
-              for (int h = 0; h < NHE; ++h) s += etb[(long long)h * ldb + fc] * s_X[h * 16 + i16];
-              dacc += s;
+              for (int h = 0; h < NHE; ++h) s += etb[(long long)h * ldb + fc] * s_X[h][i16];
+              double *d = out + CD.off + cc;
+              *d = accumulate ? *d + s : s;
             }
-          } else {
-#pragma unroll
-            for (int ks = 0; ks < KSE; ++ks) {
-              const int h = 4 * ks + g;
-              const double a = s_X[h * 16 + i16];
-#pragma unroll
-              for (int rt = 0; rt < RT; ++rt) {
-                if (rt < nrt) {
-                  const double b = (h < NHE) ? etb[(long long)h * ldb + fr[rt]] : 0.0;
-                  vacc[rt] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, vacc[rt], 0, 0, 0);
-                }
-              }
-            }
+            __syncthreads();
+            continue;
           }
-        }
-        // store: lane (j, g) holds V(16 rt + j, c0 + 4 q + g)
-        if (A.mode == LAUNCH_DIAG) {
-          if (g == 0 && cc < nc) dst[cc] = dacc;
-        } else {
+          // append the slice's columns to the ring.  MFMA f64 16x16x4 result
+          // layout: element (i, j) of a tile sits in lane j + 16 (i % 4),
+          // register i / 4, so lane (j, g) holds rows 16 ht + 4 qq + g of column j.
+          const int slot = (head + q + i16) & (MD_RING - 1);
+          if (i16 < ncs) {
 #pragma unroll
-          for (int rt = 0; rt < RT; ++rt) {
-            const int r = 16 * rt + i16;
-            if (rt < nrt && r < nr) {
+            for (int ht = 0; ht < HT; ++ht)
 #pragma unroll
-              for (int q = 0; q < 4; ++q) {
-                const int c = c0 + 4 * q + g;
-                if (c < nc) dst[r + A.ldo * c] = vacc[rt][q];
-              }
-            }
+              for (int qq = 0; qq < 4; ++qq) s_X[16 * ht + 4 * qq + g][slot] = xacc[ht][qq];
+            if (g == 0) s_qbase[slot] = A.trans ? CD.off + cc : CD.off + A.ldo * cc;
           }
+          q += ncs;
+          __syncthreads();
+          if (q >= 16) flush(16);
         }
       }
+      if (q > 0) flush(q);
     }
   }
 }

@@ -53,6 +53,21 @@ module gigpu_c
       real(c_double), value :: kappa
       real(c_double), intent(out) :: out(*)
     end function
+    integer(c_int) function gi_eri_pair_block(ctx, nrow_pairs, row_pairs, ncol_pairs, col_pairs, kappa, out, ldo) &
+        bind(C, name="gi_eri_pair_block")
+      import :: c_int, c_ptr, c_int64_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int64_t), value :: nrow_pairs, ncol_pairs, ldo
+      integer(c_int64_t), intent(in) :: row_pairs(2, *), col_pairs(2, *)
+      real(c_double), value :: kappa
+      real(c_double), intent(out) :: out(ldo, *)
+    end function
+    integer(c_int) function gi_set_option(ctx, name, value) bind(C, name="gi_set_option")
+      import :: c_int, c_ptr, c_int64_t, c_char
+      type(c_ptr), value :: ctx
+      character(kind=c_char), intent(in) :: name(*)
+      integer(c_int64_t), value :: value
+    end function
     integer(c_int) function gi_chol_build(ctx, tau, kappa, max_block_dim, vec_block, flags) &
         bind(C, name="gi_chol_build")
       import :: c_int, c_ptr, c_int64_t, c_double
@@ -98,6 +113,24 @@ module gigpu_c
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: ctx
       real(c_double), intent(out) :: t(14)
+    end function
+    integer(c_int) function gi_chol_rkpq_device(ctx, ptr) bind(C, name="gi_chol_rkpq_device")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      type(c_ptr), intent(out) :: ptr
+    end function
+    integer(c_int) function gi_eri_class_stats(ctx, cap, rows, n) bind(C, name="gi_eri_class_stats")
+      import :: c_int, c_ptr, c_int64_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int64_t), value :: cap
+      real(c_double), intent(out) :: rows(5, *)
+      integer(c_int64_t), intent(out) :: n
+    end function
+    integer(c_int) function gi_mo_transform_device(ctx, CA, lda, a0, a1, CB, ldb, b0, b1, max_buffer_mb, flags, &
+        Rkab) bind(C, name="gi_mo_transform_device")
+      import :: c_int, c_ptr, c_int64_t
+      type(c_ptr), value :: ctx, CA, CB, Rkab
+      integer(c_int64_t), value :: lda, a0, a1, ldb, b0, b1, max_buffer_mb, flags
     end function
     integer(c_int) function gi_mo_transform(ctx, CA, lda, a0, a1, CB, ldb, b0, b1, max_buffer_mb, flags, Rkab) &
         bind(C, name="gi_mo_transform")

@@ -16,7 +16,8 @@ LIB_PATH = os.path.join(HERE, "libgigpu.so")
 EXPORTS = ["gi_create", "gi_destroy", "gi_last_error", "gi_set_basis", "gi_eri_quartets",
            "gi_chol_build", "gi_chol_accuracy_tau", "gi_chol_dims", "gi_chol_meta", "gi_chol_rkpq",
            "gi_chol_rkpq_device", "gi_chol_inv_l", "gi_chol_vdiag", "gi_chol_timings",
-           "gi_mo_transform", "gi_mo_transform_device", "gi_eri_class_stats"]
+           "gi_mo_transform", "gi_mo_transform_device", "gi_eri_class_stats", "gi_eri_pair_block",
+           "gi_set_option"]
 
 CHOL_DECODE_FIXED = 1
 CHOL_SKIP_VECTORS = 2
@@ -58,6 +59,8 @@ def load():
     L.gi_chol_vdiag.argtypes = [C.c_void_p, dptr]
     L.gi_chol_timings.argtypes = [C.c_void_p, dptr]
     L.gi_eri_class_stats.argtypes = [C.c_void_p, i64, dptr, iptr]
+    L.gi_eri_pair_block.argtypes = [C.c_void_p, i64, iptr, i64, iptr, C.c_double, dptr, i64]
+    L.gi_set_option.argtypes = [C.c_void_p, C.c_char_p, i64]
     L.gi_mo_transform.argtypes = [C.c_void_p, dptr, i64, i64, i64, dptr, i64, i64, i64, i64, i64, dptr]
     L.gi_mo_transform_device.argtypes = [C.c_void_p, C.c_void_p, i64, i64, i64, C.c_void_p, i64, i64, i64, i64,
                                          i64, C.c_void_p]
@@ -122,6 +125,20 @@ class Context:
         for (na, nb, nc, nd), o, n in zip(dims, offs, sizes):
             blocks.append(out[o:o + n].reshape(nd, nc, nb, na).transpose(3, 2, 1, 0))
         return blocks
+
+    def eri_pair_block(self, row_pairs, col_pairs, kappa=0.0):
+        """Dense (pq|rs) block: rows = orbital pairs of the row shell pairs
+        (0-based (A, B), a fastest), columns likewise."""
+        rp = np.ascontiguousarray(np.asarray(row_pairs, dtype=np.int64).reshape(-1, 2))
+        cp = np.ascontiguousarray(np.asarray(col_pairs, dtype=np.int64).reshape(-1, 2))
+        nr = sum(self.basis.shell_nang(int(a)) * self.basis.shell_nang(int(b)) for a, b in rp)
+        nc = sum(self.basis.shell_nang(int(a)) * self.basis.shell_nang(int(b)) for a, b in cp)
+        out = np.zeros(nr * nc)
+        _check(self.L.gi_eri_pair_block(self.h, len(rp), _p(rp, i64), len(cp), _p(cp, i64), kappa, _p(out), nr))
+        return out.reshape(nc, nr).T
+
+    def set_option(self, name, value):
+        _check(self.L.gi_set_option(self.h, name.encode(), int(value)))
 
     # -- Cholesky --------------------------------------------------------
     def chol_build(self, tau=1e-6, kappa=0.0, max_block_dim=4000, vec_block=500, flags=0):

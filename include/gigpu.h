@@ -73,6 +73,27 @@ int gi_set_basis(gi_context *ctx, int64_t natoms, const double *coords, int64_t 
 int gi_eri_quartets(gi_context *ctx, int64_t nq, const int64_t *quartets, double kappa,
                     const int64_t *offsets, double *out, int64_t out_len);
 
+/* Dense block of (pq|rs) between two lists of shell pairs, the product form
+ * of the ERI launches behind chol2_M / chol2_Vabrs (TwoStepCholesky_Step1.f90:733-856,
+ * TwoStepCholesky_Step2.f90:242-363) without screening or selection: row pair
+ * i = (row_pairs[2i], row_pairs[2i+1]) (0-based shells) contributes its
+ * n0*n1 orbital pairs (a fastest) as consecutive rows, column pairs likewise
+ * as consecutive columns; out is column-major with leading dimension ldo.
+ * Host pointers. */
+int gi_eri_pair_block(gi_context *ctx, int64_t nrow_pairs, const int64_t *row_pairs, int64_t ncol_pairs,
+                      const int64_t *col_pairs, double kappa, double *out, int64_t ldo);
+
+/* Tuning options of the ERI launch layer (no reference counterpart):
+ *   "eri_min_tiles"  keep at least this many work tiles per MFMA launch (8192)
+ *   "eri_max_kt"     most ket pairs stacked in one MFMA tile (32)
+ *   "eri_lane"       1: small uncontracted classes on the lane-per-quartet
+ *                    kernels, 0: every class on the MFMA kernels (1)
+ *   "eri_trans"      1: MFMA launches put the side with more rows per pair
+ *                    on the 16-row tiles, 0: never transpose (1)
+ * Unknown names fail.  Results do not depend on these settings beyond
+ * floating-point rounding. */
+int gi_set_option(gi_context *ctx, const char *name, int64_t value);
+
 /* Two-step Cholesky decomposition of the Coulomb matrix, the work of
  * chol_gammcor_Rkpq -> chol_gammcor_CoulombMatrix (Cholesky_Gammcor.f90:335-441):
  * Step 1 (pivots), Step 2 (Inv_L) and the full-dimension vectors Rkpq, all

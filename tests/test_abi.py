@@ -62,3 +62,10 @@ def test_fortran_interface(tmp_path):
     r = subprocess.run([exe], cwd=tmp_path, capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "fortran abi ok" in r.stdout
+
+
+def test_fortran_interface_declares_every_function():
+    src = open(os.path.join(ROOT, "gammcor-integrals_amd", "fortran", "gigpu_interface.f90")).read()
+    bound = set(re.findall(r'bind\(C,\s*name="(gi_[a-z0-9_]+)"\)', src))
+    missing = [n for n in header_functions() if n not in bound]
+    assert not missing, missing

@@ -67,3 +67,55 @@ def test_many_random_quartets(setup):
     quartets = [list(rng.integers(0, b.nshells, 4)) for _ in range(1500)]
     for q, G in zip(quartets, ctx.eri_quartets(quartets)):
         assert np.abs(G - o.eri(*q)).max() <= TOL
+
+
+def _pairs(b, rng, lpairs, n):
+    by_l = {l: [i for i in range(b.nshells) if b.shell_l(i) == l] for l in range(4)}
+    out = []
+    for k in range(n):
+        l0, l1 = lpairs[k % len(lpairs)]
+        out.append((int(rng.choice(by_l[l0])), int(rng.choice(by_l[l1]))))
+    return out
+
+
+ALL_L = [(a, c) for a in range(4) for c in range(4)]
+THIN = [(0, 0), (1, 0), (0, 1), (1, 1)]
+THICK = [(3, 3), (3, 2), (2, 3), (2, 2), (3, 1)]
+
+
+@pytest.mark.parametrize("rows,cols", [(ALL_L, ALL_L), (THICK, THIN), (THIN, THICK)],
+                         ids=["mixed", "thin-kets", "thin-bras"])
+@pytest.mark.parametrize("lane,trans,kt", [(1, 1, 32), (0, 1, 32), (0, 0, 32), (0, 1, 1)],
+                         ids=["default", "mfma-only", "no-transpose", "one-ket-tiles"])
+def test_pair_block_launch_shapes(setup, rows, cols, lane, trans, kt):
+    """The product launches with every kernel choice: ket stacking across
+    pairs of one tile, transposed orientation, MFMA kernels for the small
+    classes, single-ket tiles."""
+    b, o, ctx = setup
+    rng = np.random.default_rng(7)
+    rp, cp = _pairs(b, rng, rows, 37), _pairs(b, rng, cols, 41)
+    try:
+        ctx.set_option("eri_min_tiles", 1)
+        ctx.set_option("eri_max_kt", kt)
+        ctx.set_option("eri_lane", lane)
+        ctx.set_option("eri_trans", trans)
+        V = ctx.eri_pair_block(rp, cp)
+    finally:
+        ctx.set_option("eri_min_tiles", 8192)
+        ctx.set_option("eri_max_kt", 32)
+        ctx.set_option("eri_lane", 1)
+        ctx.set_option("eri_trans", 1)
+    nang = b.shell_nang
+    r0 = 0
+    worst = 0.0
+    for (A, B) in rp:
+        nr = nang(A) * nang(B)
+        c0 = 0
+        for (Cs, D) in cp:
+            nc = nang(Cs) * nang(D)
+            # row a + na b, column c + nc d
+            ref = o.eri(A, B, Cs, D).transpose(1, 0, 3, 2).reshape(nr, nc)
+            worst = max(worst, float(np.abs(V[r0:r0 + nr, c0:c0 + nc] - ref).max()))
+            c0 += nc
+        r0 += nr
+    assert worst <= TOL, worst
