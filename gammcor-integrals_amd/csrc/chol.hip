@@ -352,7 +352,9 @@ void chol_build(Context &C, double tau, double kappa, long long max_block_dim, l
   S.valid = false;
   S.inv_l.release();
   S.rkpq.release();
-  S.mo_map.release();
+  S.mo_adj_off.release();
+  S.mo_adj_col.release();
+  S.mo_adj_p.release();
   S.timings.clear();
   S.eri_kernel_ms.clear();
   S.eri_quartets.clear();

@@ -54,7 +54,10 @@ struct CholState {
   std::vector<double> vdiag;
   DBuf<double> inv_l;   // nvecs x nvecs
   DBuf<double> rkpq;    // nvecs x max_subset_dim x nsubsets
-  DBuf<long long> mo_map;   // AO pair (p + nao q) -> Rkpq column, -1 if screened
+  // MO transform pair lists: for AO q, entries mo_adj_off[q]..mo_adj_off[q+1]-1
+  // hold the partner AO p and the Rkpq column of the orbital pair {p, q}
+  DBuf<long long> mo_adj_off, mo_adj_col;
+  DBuf<int> mo_adj_p;
   std::map<std::string, double> timings;
   std::map<std::string, double> eri_kernel_ms;   // per phase ERI kernel time
   std::map<std::string, long long> eri_quartets; // per phase quartet counts

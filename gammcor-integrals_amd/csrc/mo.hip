@@ -1,13 +1,15 @@
 // AO -> MO transform of the Cholesky vectors on the device,
 //   Rkab(k, a, b) = sum_{p,q} L^k_{pq} CA(p, a) CB(q, b),
-// the operation of chol_gammcor_MOTransf_2 (Cholesky_Gammcor.f90:143-332),
-// recast from the reference's rank-1 updates (dger) and matrix-vector
-// products (dgemv) into two FP64 GEMMs per vector block:
-//   1. unpack R(k, pq) of a block of vectors into dense symmetric L^k
-//      (pairs screened out of BASE are zero, as in the reference's Step 1
-//      loop which visits BASE pairs only),
-//   2. W(k, a, p) = sum_q L^k(q, p) CA(q, a)   strided-batched DGEMM over p,
-//   3. Rkab(k a, b) = sum_p W(k a, p) CB(p, b)  one DGEMM.
+// the operation of chol_gammcor_MOTransf_2 (Cholesky_Gammcor.f90:143-332).
+// The reference's first step adds R_k(pq) CA(p,a) into W(:,a,q) and
+// R_k(pq) CA(q,a) into W(:,a,p) pair by pair (two dger per orbital pair,
+// :215-292).  Here the same sum runs pair-list by pair-list: for every AO q,
+// the orbital pairs that contain q (p, column of Rkpq) form a gathered
+// product
+//   W(k, a, q) = sum_{j in pairs(q)} R(k, col_j) CA(p_j, a)
+// on FP64 MFMA (k_mo_half), reading Rkpq in place (no dense L^k), then
+//   Rkab(k a, b) = sum_q W(k a, q) CB(q, b)
+// is one DGEMM (the reference's second step, dgemv per b, :295-332).
 #include <algorithm>
 #include <cmath>
 #include "gi_context.h"
@@ -22,14 +24,69 @@ namespace gi {
                          std::to_string(__LINE__)};                                              \
   } while (0)
 
-// L[kk + kb (q + nao p)] = Rkpq[(k0 + kk) + nvec * map[q + nao p]] or 0
-__global__ void k_unpack(double *__restrict__ L, const double *__restrict__ R, const long long *__restrict__ map,
-                         long long nvec, long long k0, long long kb, long long nao) {
-  long long kk = threadIdx.x + (long long)blockIdx.x * blockDim.x;
-  long long qp = blockIdx.y + (long long)gridDim.y * blockIdx.z;
-  if (kk >= kb || qp >= nao * nao) return;
-  long long idx = map[qp];
-  L[kk + kb * qp] = idx >= 0 ? R[(k0 + kk) + nvec * idx] : 0.0;
+typedef double mo_d4 __attribute__((ext_vector_type(4)));
+
+// W(k, a, q) for one AO q (blockIdx.x), 64 vectors k0.. (blockIdx.y) and 64
+// MO columns a0.. (blockIdx.z).  MFMA f64 16x16x4: A = CA^T rows of the
+// partners p (a x j), B = the Rkpq columns of the pairs (j x k), so the
+// result tiles are (a, k) with k fastest across lanes (coalesced stores).
+// CT(a, p) = CA(p, a0 + a) with leading dimension ldct (>= NA rounded up to 64).
+__global__ void __launch_bounds__(64) k_mo_half(const double *__restrict__ R, long long nvec,
+                                                const long long *__restrict__ adj_off,
+                                                const long long *__restrict__ adj_col,
+                                                const int *__restrict__ adj_p, const double *__restrict__ CT,
+                                                long long ldct, long long NA, double *__restrict__ W) {
+  const int lane = threadIdx.x, i16 = lane & 15, g = lane >> 4;
+  const long long q = blockIdx.x;
+  const long long k0 = (long long)blockIdx.y * 64, a0 = (long long)blockIdx.z * 64;
+  const long long j0 = adj_off[q], j1 = adj_off[q + 1];
+  mo_d4 acc[4][4];
+#pragma unroll
+  for (int at = 0; at < 4; ++at)
+#pragma unroll
+    for (int kt = 0; kt < 4; ++kt) acc[at][kt] = mo_d4{0.0, 0.0, 0.0, 0.0};
+  for (long long j = j0; j < j1; j += 4) {
+    const long long jj = j + g;           // this lane's partner in the k-step
+    const bool ok = jj < j1;
+    const long long col = ok ? adj_col[jj] : 0;
+    const int p = ok ? adj_p[jj] : 0;
+    double av[4], bv[4];
+#pragma unroll
+    for (int at = 0; at < 4; ++at) {
+      const long long a = a0 + 16 * at + i16;
+      av[at] = (ok && a < NA) ? CT[a + ldct * p] : 0.0;
+    }
+#pragma unroll
+    for (int kt = 0; kt < 4; ++kt) {
+      const long long k = k0 + 16 * kt + i16;
+      bv[kt] = (ok && k < nvec) ? R[k + nvec * col] : 0.0;
+    }
+#pragma unroll
+    for (int at = 0; at < 4; ++at)
+#pragma unroll
+      for (int kt = 0; kt < 4; ++kt) acc[at][kt] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[at], bv[kt], acc[at][kt], 0, 0, 0);
+  }
+  // element (i, j) of a result tile sits in lane j + 16 (i % 4), register i / 4
+#pragma unroll
+  for (int at = 0; at < 4; ++at)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const long long a = a0 + 16 * at + 4 * r + g;
+      if (a >= NA) continue;
+#pragma unroll
+      for (int kt = 0; kt < 4; ++kt) {
+        const long long k = k0 + 16 * kt + i16;
+        if (k < nvec) W[k + nvec * (a + NA * q)] = acc[at][kt][r];
+      }
+    }
+}
+
+// CT(a, p) = CA(p, a0 + a), zero-padded to ldct rows
+__global__ void k_mo_ct(double *__restrict__ CT, long long ldct, const double *__restrict__ CA, long long lda,
+                        long long a0, long long NA, long long nao) {
+  const long long a = threadIdx.x + (long long)blockIdx.x * blockDim.x;
+  const long long p = blockIdx.y;
+  if (a < ldct && p < nao) CT[a + ldct * p] = a < NA ? CA[p + lda * (a0 - 1 + a)] : 0.0;
 }
 
 // zero Rkab(:, a, :) for the a columns in `cols`
@@ -60,43 +117,47 @@ void mo_transform(Context &C, const double *dCA, long long lda, long long a0, lo
     long long a0k = (k - 1) * maxnk + a0, a1k = std::min(k * maxnk, a1);
     for (long long a = a0k; a <= a1k; ++a) done[a - a0] = 1;
   }
-  // AO pair -> Rkpq column map (cached per Cholesky build)
-  if (!S.mo_map.p) {
-    std::vector<long long> map(nao * nao, -1);
+  // pair lists per AO (cached per Cholesky build): for AO q every BASE
+  // orbital pair (p, q) or (q, p) with its Rkpq column (X - 1) msd + position
+  if (!S.mo_adj_off.p) {
+    std::vector<std::vector<std::pair<int, long long>>> adj(nao);
     for (long long s = 0; s < S.nshell_pairs; ++s) {
-      int A = (int)S.shell_pairs[2 * s] - 1, B = (int)S.shell_pairs[2 * s + 1] - 1;
-      long long X = S.shell_pair_loc[3 * s + 2], col = S.shell_pair_loc[3 * s + 1] - 1;
-      int na = C.shell_nang[A], nb = C.shell_nang[B];
-      long long base = (X - 1) * msd + col;
-      long long q = 0;
+      const int A = (int)S.shell_pairs[2 * s] - 1, B = (int)S.shell_pairs[2 * s + 1] - 1;
+      const long long X = S.shell_pair_loc[3 * s + 2], pos0 = S.shell_pair_loc[3 * s + 1] - 1;
+      const int na = C.shell_nang[A], nb = C.shell_nang[B];
+      long long qq = 0;
       for (int b = 0; b < nb; ++b)
         for (int a = (A == B ? b : 0); a < na; ++a) {
-          long long p1 = C.shell_loc[A] + a, p2 = C.shell_loc[B] + b;
-          map[p1 + nao * p2] = base + q;
-          map[p2 + nao * p1] = base + q;
-          ++q;
+          const int p1 = C.shell_loc[A] + a, p2 = C.shell_loc[B] + b;
+          const long long col = (X - 1) * msd + pos0 + qq;
+          adj[p2].push_back({p1, col});
+          if (p1 != p2) adj[p1].push_back({p2, col});
+          ++qq;
         }
     }
-    S.mo_map.upload(map, C.stream);
+    std::vector<long long> off(nao + 1, 0), cols;
+    std::vector<int> ps;
+    for (long long q = 0; q < nao; ++q) {
+      off[q + 1] = off[q] + (long long)adj[q].size();
+      for (auto &e : adj[q]) {
+        ps.push_back(e.first);
+        cols.push_back(e.second);
+      }
+    }
+    S.mo_adj_off.upload(off, C.stream);
+    S.mo_adj_col.upload(cols, C.stream);
+    S.mo_adj_p.upload(ps, C.stream);
   }
-  // vector blocks: keep the dense L^k block under ~4 GiB
-  long long kb = std::max<long long>(1, (long long)((4.0 * (1ull << 30)) / (8.0 * (double)nao * nao)));
-  kb = std::min(kb, nvec);
-  if (kb > 64) kb -= kb % 64;
-  DBuf<double> dL, dW;
-  dL.alloc((size_t)kb * nao * nao);
+  const long long ldct = (NA + 63) / 64 * 64;
+  DBuf<double> dCT, dW;
+  dCT.alloc((size_t)ldct * nao);
   dW.alloc((size_t)nvec * NA * nao);
+  hipLaunchKernelGGL(k_mo_ct, dim3((unsigned)(ldct / 64), (unsigned)nao), dim3(64), 0, C.stream, dCT.p, ldct, dCA,
+                     lda, a0, NA, nao);
+  hipLaunchKernelGGL(k_mo_half, dim3((unsigned)nao, (unsigned)((nvec + 63) / 64), (unsigned)(ldct / 64)), dim3(64), 0,
+                     C.stream, S.rkpq.p, nvec, S.mo_adj_off.p, S.mo_adj_col.p, S.mo_adj_p.p, dCT.p, ldct, NA, dW.p);
+  GI_HIP(hipGetLastError());
   const double one = 1.0, zero = 0.0;
-  for (long long k0 = 0; k0 < nvec; k0 += kb) {
-    long long kbl = std::min(kb, nvec - k0);
-    long long nqp = nao * nao;
-    dim3 grid((unsigned)((kbl + 63) / 64), (unsigned)std::min<long long>(nqp, 65535),
-              (unsigned)((nqp + 65534) / 65535));
-    hipLaunchKernelGGL(k_unpack, grid, dim3(64), 0, C.stream, dL.p, S.rkpq.p, S.mo_map.p, nvec, k0, kbl, nao);
-    RB_CHECK(rocblas_dgemm_strided_batched(C.blas, rocblas_operation_none, rocblas_operation_none, kbl, NA, nao,
-                                           &one, dL.p, kbl, (rocblas_stride)(kbl * nao), dCA + (a0 - 1) * lda, lda, 0,
-                                           &zero, dW.p + k0, nvec, (rocblas_stride)(nvec * NA), nao));
-  }
   RB_CHECK(rocblas_dgemm(C.blas, rocblas_operation_none, rocblas_operation_none, nvec * NA, NB, nao, &one, dW.p,
                          nvec * NA, dCB + (b0 - 1) * ldb, ldb, &zero, dRkab, nvec * NA));
   if (!(flags & 1)) {
