@@ -360,6 +360,7 @@ void chol_build(Context &C, double tau, double kappa, long long max_block_dim, l
   S.eri_quartets.clear();
   S.tau = tau;
   S.kappa = kappa;
+  S.nmacro = 0;
   C.time_eri = (flags & 4) != 0;
   C.eri_ms_acc = 0;
   C.eri_quartets_acc = 0;
@@ -874,25 +875,28 @@ void chol_build(Context &C, double tau, double kappa, long long max_block_dim, l
     per_group = std::min(per_group, nsub);
     DBuf<double> dW;
     dW.alloc(slab * per_group);
-    std::vector<PairDesc> cd(npsp);
-    std::vector<int> cols(npsp);
-    for (long long k = 0; k < npsp; ++k) {
-      long long ab = psp[k];
-      cd[k] = PairDesc{spA[ab], spB[ab], PAIR_SEL, (int)(psploc[k] - 1), (int)pspdim[k], 0, msd * (psploc[k] - 1)};
-      cols[k] = (int)k;
-    }
     double t_vab = 0, t_gemm = 0;
     for (long long X0 = 1; X0 <= nsub; X0 += per_group) {
       long long X1 = std::min(nsub, X0 + per_group - 1);
       long long ng = X1 - X0 + 1;
+      // Vabrs of the group's subsets stacked into one (ng msd) x NVecs matrix:
+      // row (X - X0) msd + subset position, column = pivot
+      const long long ldw = msd * ng;
       GI_HIP(hipMemsetAsync(dW.p, 0, sizeof(double) * slab * ng, C.stream));
+      std::vector<PairDesc> cd(npsp);
+      std::vector<int> cols(npsp);
+      for (long long k = 0; k < npsp; ++k) {
+        long long ab = psp[k];
+        cd[k] = PairDesc{spA[ab], spB[ab], PAIR_SEL, (int)(psploc[k] - 1), (int)pspdim[k], 0, ldw * (psploc[k] - 1)};
+        cols[k] = (int)k;
+      }
       std::vector<PairDesc> rd;
       std::vector<int> rows;
       for (long long X = X0; X <= X1; ++X) {
         if (sb0[X] <= 0) continue;
         for (long long s = sb0[X]; s <= sb1[X]; ++s) {
           rd.push_back(PairDesc{spA[s], spB[s], spA[s] == spB[s] ? PAIR_SAME : 0, 0, 0, 0,
-                                (long long)((X - X0) * slab) + loc_sub[s] - 1});
+                                (X - X0) * msd + loc_sub[s] - 1});
           rows.push_back((int)rd.size() - 1);
         }
       }
@@ -902,23 +906,23 @@ void chol_build(Context &C, double tau, double kappa, long long max_block_dim, l
         EriRequest R;
         R.rdesc = &rd; R.cdesc = &cd;
         R.rows = &rows; R.cols = &cols; R.mode = LAUNCH_PRODUCT;
-        R.out = dW.p; R.ldo = msd; R.d_sel = dsel.p; R.kappa = kappa;
+        R.out = dW.p; R.ldo = ldw; R.d_sel = dsel.p; R.kappa = kappa;
         eri_run(C, R);
       }
       t_vab += now() - tv;
       double tg = now();
-      // Inv_L is lower triangular: block row r of Rkpq only needs the first
-      // r1 columns of Inv_L, so the product is one GEMM per block row
-      // ((nblk + 1) / (2 nblk) of the dense FLOPs)
+      // Rkpq(:, :, X0..X1) is one NVecs x (ng msd) matrix (subsets are
+      // consecutive in Rkpq), = Inv_L Vabrs^T.  Inv_L is lower triangular:
+      // block row r only needs the first r1 columns of Inv_L, so the product
+      // is one GEMM per block row ((nblk + 1) / (2 nblk) of the dense FLOPs)
       const double one = 1.0, zero = 0.0;
       const long long nblk = std::max<long long>(1, std::min<long long>(8, nvec / 512));
       for (long long rb = 0; rb < nblk; ++rb) {
         const long long r0 = nvec * rb / nblk, r1 = nvec * (rb + 1) / nblk;
         if (r1 <= r0) continue;
-        ROCBLAS_CHECK(rocblas_dgemm_strided_batched(
-            C.blas, rocblas_operation_none, rocblas_operation_transpose, r1 - r0, msd, r1, &one, S.inv_l.p + r0,
-            nvec, 0, dW.p, msd, (rocblas_stride)slab, &zero, S.rkpq.p + (size_t)(X0 - 1) * nvec * msd + r0, nvec,
-            (rocblas_stride)((size_t)nvec * msd), ng));
+        ROCBLAS_CHECK(rocblas_dgemm(C.blas, rocblas_operation_none, rocblas_operation_transpose, r1 - r0, ldw, r1,
+                                    &one, S.inv_l.p + r0, nvec, dW.p, ldw, &zero,
+                                    S.rkpq.p + (size_t)(X0 - 1) * nvec * msd + r0, nvec));
       }
       GI_HIP(hipStreamSynchronize(C.stream));
       t_gemm += now() - tg;
