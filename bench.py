@@ -60,6 +60,19 @@ class Hip:
         self.L.hipDeviceSynchronize()
 
 
+def profiled_traffic():
+    """HBM bytes per ERI launch measured by the committed counter profile of
+    this workload, or None."""
+    path = os.path.join(ROOT, "profiles", "eri_kernels_h2o16_hbm_bytes.md")
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return int(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(seed: int):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
@@ -153,6 +166,8 @@ def main():
     quartets = d["nquartets"]
     eri_s = tm["eri_kernel"]
     eri_flops = tm["eri_flops"]
+    launches = int(tm["eri_launches"])
+    # algorithmic FP64 ops per launch / average launch duration (= total / total)
     achieved = eri_flops / eri_s / 1e12 if eri_s > 0 else None
     # Rkpq = Inv_L Vabrs^T with Inv_L lower triangular, one GEMM per block row
     nv = d["nvecs"]
@@ -184,9 +199,16 @@ def main():
         "eri_quartets": quartets,
         "phases_s": {k: tm[k] for k in ("vdiag", "step1", "eri_step1", "rtr", "micro", "step2", "vpqrs",
                                         "potrf_trtri", "vectors", "vabrs", "vectors_gemm", "eri_kernel")},
-        "roofline": {"bound": "mfma", "kernel": "ERI class kernels (FP64 VALU; MI355X FP64 vector peak = matrix peak)",
+        "rtr_tflops": tm["rtr_flops"] / tm["rtr"] / 1e12 if tm["rtr"] > 0 else None,
+        "roofline": {"bound": "mfma",
+                     "kernel": "ERI kernel family: md_kernel (FP64 MFMA) + md_lane_kernel (FP64 VALU) + k_build_et; "
+                               "MI355X FP64 vector peak = FP64 matrix peak",
                      "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": (achieved / FP64_PEAK_TFLOPS) if achieved else None, "traffic": None,
+                     "frac": (achieved / FP64_PEAK_TFLOPS) if achieved else None,
+                     "traffic": profiled_traffic(),
+                     "traffic_source": "profiles/eri_kernels_h2o16_hbm_bytes.md (HBM bytes per launch, "
+                                       "FETCH_SIZE + WRITE_SIZE scaled by a copy probe)",
+                     "launches": launches, "flops_per_launch": eri_flops / max(launches, 1),
                      "flops": eri_flops, "kernel_s": eri_s},
     }
     if not args.no_cpu_baseline:

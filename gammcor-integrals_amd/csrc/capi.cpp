@@ -309,7 +309,7 @@ int gi_chol_vdiag(gi_context *cp, double *dst) {
   })
 }
 
-int gi_chol_timings(gi_context *cp, double t[14]) {
+int gi_chol_timings(gi_context *cp, double t[16]) {
   GI_TRY({
     gi::Context &C = ctx_of(cp);
     auto &T = C.chol.timings;
@@ -328,6 +328,8 @@ int gi_chol_timings(gi_context *cp, double t[14]) {
     t[11] = g("potrf_trtri");
     t[12] = g("vabrs");
     t[13] = g("vectors_gemm");
+    t[14] = g("eri_launches");
+    t[15] = g("rtr_flops");
   })
 }
 

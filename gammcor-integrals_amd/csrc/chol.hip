@@ -365,6 +365,7 @@ void chol_build(Context &C, double tau, double kappa, long long max_block_dim, l
   C.eri_ms_acc = 0;
   C.eri_quartets_acc = 0;
   C.eri_flops_acc = 0;
+  C.eri_launches_acc = 0;
   const int nsh = C.nshells;
   const long long nao = C.nao;
   const long long max_sp = (long long)nsh * (nsh + 1) / 2;
@@ -513,7 +514,7 @@ void chol_build(Context &C, double tau, double kappa, long long max_block_dim, l
   long long nchol = 0;
   bool converged = false;
   double dmax = 0.0;
-  double t_eri1 = 0, t_rtr = 0, t_micro = 0;
+  double t_eri1 = 0, t_rtr = 0, t_micro = 0, rtr_flops = 0;
   {
     double t0 = now();
     long long nsps_cand = nsp, nops_cand = nop;
@@ -666,6 +667,7 @@ void chol_build(Context &C, double tau, double kappa, long long max_block_dim, l
                                     nchol, &alpha, dL.p, ncand, dLB.p, nops_batch, &beta, dM.p, ncand));
         GI_HIP(hipStreamSynchronize(C.stream));
         t_rtr += now() - tr;
+        rtr_flops += 2.0 * (double)ncand * (double)nops_batch * (double)nchol;
       }
       // micro iterations (chol2_Lj + D update + BatchMax, TwoStepCholesky_Step1.f90:1027-1064):
       // the sequential recurrence runs on the batch rows on the device; the
@@ -741,6 +743,7 @@ void chol_build(Context &C, double tau, double kappa, long long max_block_dim, l
   }
   S.timings["eri_step1"] = t_eri1;
   S.timings["rtr"] = t_rtr;
+  S.timings["rtr_flops"] = rtr_flops;
   S.timings["micro"] = t_micro;
   S.max_residual = dmax;
   S.nvecs = nchol;
@@ -935,6 +938,7 @@ void chol_build(Context &C, double tau, double kappa, long long max_block_dim, l
   S.timings["total"] = now() - t_begin;
   S.nquartets = C.eri_quartets_acc;
   S.eri_flops = C.eri_flops_acc;
+  S.timings["eri_launches"] = (double)C.eri_launches_acc;
   S.timings["eri_kernel"] = C.eri_ms_acc / 1000.0;
   S.valid = true;
 }

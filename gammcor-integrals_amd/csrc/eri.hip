@@ -287,7 +287,7 @@ void eri_run(Context &C, const EriRequest &R) {
   C.md_idx.upload(idx, C.stream);
 
   static const bool stats = getenv("GIGPU_ERI_STATS") != nullptr;
-  long long nq_total = 0;
+  long long nq_total = 0, nlaunch = 1;   // k_build_et
   double flops_total = 0;
   auto nrow_of = [&](const PairDesc &d) {
     return (double)pair_rows_host(d, C.shell_nang[d.s0], C.shell_nang[d.s1]);
@@ -396,6 +396,7 @@ void eri_run(Context &C, const EriRequest &R) {
     }
     (lane_path ? lane_table[KE][KF] : md_table[KE][KF])(a, (int)grid, C.stream);
     GI_HIP(hipGetLastError());
+    ++nlaunch;
     if (stats) {
       GI_HIP(hipEventRecord(s1, C.stream));
       GI_HIP(hipEventSynchronize(s1));
@@ -422,6 +423,7 @@ void eri_run(Context &C, const EriRequest &R) {
   }
   C.eri_quartets_acc += nq_total;
   C.eri_flops_acc += (long long)flops_total;
+  C.eri_launches_acc += nlaunch;
   // the pair table and index buffers are reused by the next request
   GI_HIP(hipStreamSynchronize(C.stream));
 }

@@ -88,6 +88,7 @@ struct Context {
   double eri_ms_acc = 0;
   long long eri_quartets_acc = 0;
   long long eri_flops_acc = 0;
+  long long eri_launches_acc = 0;   // kernel dispatches (pair tables + quartet kernels)
   // per kernel class (key 16*E + F, E = la+lb, F = lc+ld): quartets, kernel ms, FP64 ops
   // (filled when GIGPU_ERI_STATS is set in the environment)
   std::map<int, std::array<double, 3>> class_stats;

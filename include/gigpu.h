@@ -134,8 +134,9 @@ int gi_chol_vdiag(gi_context *ctx, double *dst);
  * [3] total build, [4] ERI kernel seconds (GI_CHOL_TIME_ERI), [5] ERI FP64
  * operations, [6] Vdiag, [7] step-1 ERIs, [8] M - R^T R, [9] micro
  * iterations, [10] V(P,P) ERIs, [11] POTRF+TRTRI, [12] Vabrs ERIs,
- * [13] Inv_L V^T GEMM. */
-int gi_chol_timings(gi_context *ctx, double t[14]);
+ * [13] Inv_L V^T GEMM, [14] ERI kernel launches (count), [15] M - R^T R
+ * FP64 operations. */
+int gi_chol_timings(gi_context *ctx, double t[16]);
 
 /* Per-kernel ERI statistics (diagnostic; no reference counterpart), gathered
  * when the environment variable GIGPU_ERI_STATS is set: up to `cap` rows of
