@@ -122,6 +122,7 @@ int gi_set_basis(gi_context *cp, int64_t natoms, const double *coords, int64_t n
     GI_HIP(hipStreamSynchronize(C.stream));
     C.basis_set = true;
     C.chol.valid = false;
+    C.reset_pair_cache();
   })
 }
 

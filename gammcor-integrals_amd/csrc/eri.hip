@@ -122,6 +122,19 @@ __global__ void __launch_bounds__(64) k_build_et(const DevShell *__restrict__ sh
 }
 
 
+// packed record of entry i of a launch index list (side 0: bra list, 1: ket list)
+__global__ void k_pack_recs(const PairDesc *__restrict__ rd, const PairDesc *__restrict__ cd,
+                            const int *__restrict__ idx, const unsigned char *__restrict__ side, long long n,
+                            const PairHdr *__restrict__ hdr, const PrimRec *__restrict__ prim, QRec *out) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const PairDesc d = side[i] ? cd[idx[i]] : rd[idx[i]];
+  const PairHdr h = hdr[d.pt];
+  const PrimRec p = prim[h.prim0];
+  const int nrow = (d.flags & PAIR_SEL) ? d.nsel : ((d.flags & PAIR_SAME) ? h.n0 * (h.n0 + 1) / 2 : h.n0 * h.n1);
+  out[i] = QRec{p.px, p.py, p.pz, p.p, p.et, d.off, nrow, d.flags, d.sel0, h.n0, h.n1, h.nprim, h.prim0, 0};
+}
+
 using LaunchFn = void (*)(const MdArgs &, int grid, hipStream_t);
 
 template <int E, int F>
@@ -193,15 +206,13 @@ static int pair_rows_host(const PairDesc &d, int n0, int n1) {
 void eri_run(Context &C, const EriRequest &R) {
   init_md_table();
   std::vector<PairDesc> rd = *R.rdesc, cd = *R.cdesc;
-  // ---- pair table
-  std::unordered_map<long long, int> pt_of;
-  std::vector<PairHdr> hdr;
+  // ---- pair tables: look up the basis-level cache, build the missing ones
   std::vector<EtJob> jobs;
-  long long et_total = 0;
+  const size_t nhdr0 = C.md_hdr_host.size();
   auto pair_entry = [&](PairDesc &d) {
     const long long key = (long long)d.s0 * C.nshells + d.s1;
-    auto it = pt_of.find(key);
-    if (it != pt_of.end()) {
+    auto it = C.md_pt.find(key);
+    if (it != C.md_pt.end()) {
       d.pt = it->second;
       return;
     }
@@ -213,15 +224,15 @@ void eri_run(Context &C, const EriRequest &R) {
     }
     const int n0 = C.shell_nang[d.s0], n1 = C.shell_nang[d.s1];
     const int np0 = C.shell_np[d.s0], np1 = C.shell_np[d.s1];
-    PairHdr h{n0, n1, np0 * np1, (int)jobs.size()};
+    PairHdr h{n0, n1, np0 * np1, (int)C.md_prim_used};
     for (int ib = 0; ib < np1; ++ib)
       for (int ia = 0; ia < np0; ++ia) {
-        jobs.push_back(EtJob{d.s0, d.s1, ia, ib, (int)jobs.size(), 0, et_total});
-        et_total += (long long)nherm(l0 + l1) * n0 * n1;
+        jobs.push_back(EtJob{d.s0, d.s1, ia, ib, (int)C.md_prim_used++, 0, C.md_et_used});
+        C.md_et_used += (long long)nherm(l0 + l1) * n0 * n1;
       }
-    d.pt = (int)hdr.size();
-    pt_of[key] = d.pt;
-    hdr.push_back(h);
+    d.pt = (int)C.md_hdr_host.size();
+    C.md_pt[key] = d.pt;
+    C.md_hdr_host.push_back(h);
   };
   for (int r : *R.rows) pair_entry(rd[r]);
   for (int c : *R.cols) pair_entry(cd[c]);
@@ -233,22 +244,28 @@ void eri_run(Context &C, const EriRequest &R) {
     const PairDesc &d = cd[c];
     if (pair_rows_host(d, C.shell_nang[d.s0], C.shell_nang[d.s1]) <= 0) throw Error{8, "empty ket pair"};
   }
-  if (jobs.empty()) return;
-  C.md_hdr.upload(hdr, C.stream);
-  C.md_jobs.upload(jobs, C.stream);
-  C.md_prim.alloc(jobs.size());
-  C.md_et.alloc((size_t)et_total);
-  C.md_rdesc.upload(rd, C.stream);
-  C.md_cdesc.upload(cd, C.stream);
+  if (R.rows->empty() || R.cols->empty()) return;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (C.time_eri) {
     GI_HIP(hipEventCreate(&e0));
     GI_HIP(hipEventCreate(&e1));
+  }
+  long long nlaunch = 0;
+  if (C.md_hdr_host.size() != nhdr0) C.md_hdr.upload(C.md_hdr_host, C.stream);
+  if (!jobs.empty()) {
+    C.md_prim.grow_keep((size_t)C.md_prim_used, C.stream);
+    C.md_et.grow_keep((size_t)C.md_et_used, C.stream);
+    C.md_jobs.upload(jobs, C.stream);
+    if (C.time_eri) GI_HIP(hipEventRecord(e0, C.stream));
+    hipLaunchKernelGGL(k_build_et, dim3((unsigned)jobs.size()), dim3(64), 0, C.stream, C.dshells.p, C.dexps.p,
+                       C.dcntr.p, C.dnorms.p, C.spher, C.md_jobs.p, C.md_prim.p, C.md_et.p);
+    GI_HIP(hipGetLastError());
+    ++nlaunch;
+  } else if (C.time_eri) {
     GI_HIP(hipEventRecord(e0, C.stream));
   }
-  hipLaunchKernelGGL(k_build_et, dim3((unsigned)jobs.size()), dim3(64), 0, C.stream, C.dshells.p, C.dexps.p,
-                     C.dcntr.p, C.dnorms.p, C.spher, C.md_jobs.p, C.md_prim.p, C.md_et.p);
-  GI_HIP(hipGetLastError());
+  C.md_rdesc.upload(rd, C.stream);
+  C.md_cdesc.upload(cd, C.stream);
 
   // ---- bins by E (rows) and F (columns)
   auto E_of = [&](const PairDesc &d) { return C.shell_l[d.s0] + C.shell_l[d.s1]; };
@@ -284,10 +301,22 @@ void eri_run(Context &C, const EriRequest &R) {
     coff[i] = idx.size();
     idx.insert(idx.end(), launches[i].cols.begin(), launches[i].cols.end());
   }
+  std::vector<unsigned char> side(idx.size(), 0);
+  for (size_t i = 0; i < launches.size(); ++i)
+    std::fill(side.begin() + coff[i], side.begin() + coff[i] + launches[i].cols.size(), (unsigned char)1);
   C.md_idx.upload(idx, C.stream);
+  C.md_side.upload(side, C.stream);
+  C.md_rec.alloc(idx.size());
+  if (!idx.empty()) {
+    hipLaunchKernelGGL(k_pack_recs, dim3((unsigned)((idx.size() + 255) / 256)), dim3(256), 0, C.stream, C.md_rdesc.p,
+                       C.md_cdesc.p, C.md_idx.p, C.md_side.p, (long long)idx.size(), C.md_hdr.p, C.md_prim.p,
+                       C.md_rec.p);
+    GI_HIP(hipGetLastError());
+    ++nlaunch;
+  }
 
   static const bool stats = getenv("GIGPU_ERI_STATS") != nullptr;
-  long long nq_total = 0, nlaunch = 1;   // k_build_et
+  long long nq_total = 0;
   double flops_total = 0;
   auto nrow_of = [&](const PairDesc &d) {
     return (double)pair_rows_host(d, C.shell_nang[d.s0], C.shell_nang[d.s1]);
@@ -348,13 +377,10 @@ void eri_run(Context &C, const EriRequest &R) {
       }
     }
     MdArgs a{};
-    a.hdr = C.md_hdr.p;
     a.prim = C.md_prim.p;
     a.et = C.md_et.p;
-    a.rdesc = trans ? C.md_cdesc.p : C.md_rdesc.p;
-    a.cdesc = trans ? C.md_rdesc.p : C.md_cdesc.p;
-    a.rows = C.md_idx.p + (trans ? coff[i] : roff[i]);
-    a.cols = C.md_idx.p + (trans ? roff[i] : coff[i]);
+    a.rrec = C.md_rec.p + (trans ? coff[i] : roff[i]);
+    a.crec = C.md_rec.p + (trans ? roff[i] : coff[i]);
     a.nrows = nrows;
     a.ncols = ncols;
     a.mode = R.mode;

@@ -35,6 +35,8 @@ Context::~Context() {
   md_rdesc.release();
   md_cdesc.release();
   md_idx.release();
+  md_side.release();
+  md_rec.release();
   chol.inv_l.release();
   chol.rkpq.release();
   if (stream) (void)hipStreamDestroy(stream);

@@ -2,8 +2,10 @@
 // basis, the Boys table, the ERI workspace and the Cholesky results.
 #pragma once
 #include <rocblas/rocblas.h>
+#include <algorithm>
 #include <array>
 #include <map>
+#include <unordered_map>
 #include <memory>
 #include <vector>
 #include "md_kernel.h"
@@ -37,6 +39,18 @@ struct DBuf {
     if (count) GI_HIP(hipMemcpyAsync(p, src, count * sizeof(T), hipMemcpyHostToDevice, s));
   }
   void upload(const std::vector<T> &v, hipStream_t s) { upload(v.data(), v.size(), s); }
+  // grow to at least `count` elements keeping the first n (stream-ordered copy)
+  void grow_keep(size_t count, hipStream_t s) {
+    if (count <= n && p) return;
+    size_t nn = std::max(count, n + n / 2);
+    T *q = nullptr;
+    GI_HIP(hipMalloc((void **)&q, nn * sizeof(T)));
+    if (p && n) GI_HIP(hipMemcpyAsync(q, p, n * sizeof(T), hipMemcpyDeviceToDevice, s));
+    GI_HIP(hipStreamSynchronize(s));
+    release();
+    p = q;
+    n = nn;
+  }
 };
 
 constexpr int MD_EMAX = 6;   // largest l0 + l1 of a shell pair with an ERI kernel
@@ -75,13 +89,25 @@ struct Context {
   DBuf<DevShell> dshells;
   DBuf<double> dexps, dcntr, dnorms, dboys;
   bool basis_set = false;
-  // pair tables and index lists of the last ERI request (reused, grow only)
+  // ERI pair tables, cached per basis: shell pair (s0, s1) -> entry of
+  // md_hdr_host; PrimRec and ET blocks on the device grow as pairs are added
+  std::unordered_map<long long, int> md_pt;
+  std::vector<PairHdr> md_hdr_host;
+  long long md_et_used = 0, md_prim_used = 0;
   DBuf<PairHdr> md_hdr;
   DBuf<EtJob> md_jobs;
   DBuf<PrimRec> md_prim;
   DBuf<double> md_et;
+  // per request: descriptors, launch index lists, packed records
   DBuf<PairDesc> md_rdesc, md_cdesc;
   DBuf<int> md_idx;
+  DBuf<unsigned char> md_side;
+  DBuf<QRec> md_rec;
+  void reset_pair_cache() {
+    md_pt.clear();
+    md_hdr_host.clear();
+    md_et_used = md_prim_used = 0;
+  }
   CholState chol;
   // profiling of ERI launches (events on the library stream)
   bool time_eri = false;

@@ -57,6 +57,17 @@ struct EtJob {
   int prim, pad;        // PrimRec slot
   long long et;         // ET block offset
 };
+// Everything a kernel needs about one bra or ket pair of a launch, packed so
+// that a quartet costs one independent load per side (k_pack_recs): the first
+// primitive pair inline, the rest (contracted pairs) through prim0.
+struct QRec {
+  double px, py, pz, p;   // first primitive pair
+  long long et;           // its ET block
+  long long off;          // destination offset (PairDesc::off)
+  int nrow, flags, sel0, n0;
+  int n1, nprim, prim0, pad;
+};
+__host__ __device__ inline PrimRec first_prim(const QRec &q) { return PrimRec{q.px, q.py, q.pz, q.p, q.et, 0}; }
 
 constexpr int LRMAX = 4 * MAXL;                 // highest R-table order
 constexpr int NHRMAX = (LRMAX + 1) * (LRMAX + 2) * (LRMAX + 3) / 6;
@@ -133,11 +144,9 @@ struct MdShape {
 };
 
 struct MdArgs {
-  const PairHdr *hdr;
   const PrimRec *prim;
   const double *et;
-  const PairDesc *rdesc, *cdesc;
-  const int *rows, *cols;        // indices into rdesc / cdesc
+  const QRec *rrec, *crec;       // packed bra / ket pairs of the launch
   long long nrows, ncols, ntiles;
   int mode, kt;                  // LAUNCH_*, ket pairs per tile (PRODUCT)
   int trans;                     // kernel bra side = caller's ket side (store transposed)
@@ -148,15 +157,12 @@ struct MdArgs {
   const double *boys;            // (5, BOYS_NINT, 0:BOYS_MAXM)
 };
 
-// rows of a pair in the destination, and the full-block index of row r
-__device__ __forceinline__ int pair_nrow(const PairDesc &d, const PairHdr &h) {
-  return (d.flags & PAIR_SEL) ? d.nsel : ((d.flags & PAIR_SAME) ? h.n0 * (h.n0 + 1) / 2 : h.n0 * h.n1);
-}
-__device__ __forceinline__ int pair_full(const PairDesc &d, const PairHdr &h, const int *sel, int r) {
+// full-block index a + n0 b of destination row r of a pair
+__device__ __forceinline__ int pair_full(const QRec &d, const int *sel, int r) {
   if (d.flags & PAIR_SEL) return sel[d.sel0 + r];
   if (d.flags & PAIR_SAME) {
     // column-major lower triangle position r -> (a, b), a >= b
-    const int n = h.n0;
+    const int n = d.n0;
     int b = 0, rem = r;
     while (rem >= n - b) { rem -= n - b; ++b; }
     return (b + rem) + n * b;
@@ -286,21 +292,20 @@ __global__ void __launch_bounds__(64) md_kernel(MdArgs A) {
       jc0 = t;
       nk = 1;
     }
-    const PairDesc RD = A.rdesc[A.rows[ir]];
-    const PairHdr BH = A.hdr[RD.pt];
-    const int nr = pair_nrow(RD, BH);
-    const int ldb = BH.n0 * BH.n1;
+    const QRec RB = A.rrec[ir];
+    const int nr = RB.nrow;
+    const int ldb = RB.n0 * RB.n1;
     int fr[RT];
 #pragma unroll
     for (int rt = 0; rt < RT; ++rt) {
       const int r = 16 * rt + i16;
-      fr[rt] = pair_full(RD, BH, A.sel, r < nr ? r : nr - 1);
+      fr[rt] = pair_full(RB, A.sel, r < nr ? r : nr - 1);
     }
     const int nrt = (nr + 15) / 16;
-    double *const out = A.out + RD.off;
+    double *const out = A.out + RB.off;
 
-    for (int bp = 0; bp < BH.nprim; ++bp) {
-      const PrimRec PB = A.prim[BH.prim0 + bp];
+    for (int bp = 0; bp < RB.nprim; ++bp) {
+      const PrimRec PB = bp == 0 ? first_prim(RB) : A.prim[RB.prim0 + bp];
       const double *etb = A.et + PB.et;
       const bool accumulate = bp > 0;
       int head = 0, q = 0;   // ring of queued stage-1 columns
@@ -345,12 +350,11 @@ __global__ void __launch_bounds__(64) md_kernel(MdArgs A) {
       };
 
       for (int kq = 0; kq < nk; ++kq) {
-        const PairDesc CD = A.cdesc[A.cols[jc0 + kq]];
-        const PairHdr KH = A.hdr[CD.pt];
-        const int nc = pair_nrow(CD, KH);
-        const int ldk = KH.n0 * KH.n1;
-        const bool single = (KH.nprim == 1);
-        if (single) rtable<E + F>(s_R[0], s_R[1], PB, A.prim[KH.prim0], A.kappa, A.boys, lane);
+        const QRec CD = A.crec[jc0 + kq];
+        const int nc = CD.nrow;
+        const int ldk = CD.n0 * CD.n1;
+        const bool single = (CD.nprim == 1);
+        if (single) rtable<E + F>(s_R[0], s_R[1], PB, first_prim(CD), A.kappa, A.boys, lane);
 
         for (int c0 = 0; c0 < nc; c0 += 16) {
           const int cc = c0 + i16;
@@ -363,13 +367,13 @@ __global__ void __launch_bounds__(64) md_kernel(MdArgs A) {
             for (int ps = 0; ps < 2; ++ps)
 #pragma unroll
               for (int c = 0; c < MD_NCV; ++c) xv[ps][c] = 0.0;
-            for (int kp = 0; kp < KH.nprim; ++kp) {
-              const PrimRec PK = A.prim[KH.prim0 + kp];
+            for (int kp = 0; kp < CD.nprim; ++kp) {
+              const PrimRec PK = kp == 0 ? first_prim(CD) : A.prim[CD.prim0 + kp];
               if (!single) rtable<E + F>(s_R[0], s_R[1], PB, PK, A.kappa, A.boys, lane);
               const double *etk = A.et + PK.et;
               for (int e = lane; e < NHF * MD_NCV; e += 64) {
                 const int k = e / MD_NCV, c = e - k * MD_NCV;
-                s_E[k][c] = (c < ncs) ? etk[(long long)k * ldk + pair_full(CD, KH, A.sel, c0 + c)] : 0.0;
+                s_E[k][c] = (c < ncs) ? etk[(long long)k * ldk + pair_full(CD, A.sel, c0 + c)] : 0.0;
               }
               __syncthreads();
               const double *Rf = s_R[0];
@@ -406,12 +410,12 @@ __global__ void __launch_bounds__(64) md_kernel(MdArgs A) {
             if (q >= 16) flush(16);
             continue;
           }
-          const int fc = pair_full(CD, KH, A.sel, cc < nc ? cc : nc - 1);
+          const int fc = pair_full(CD, A.sel, cc < nc ? cc : nc - 1);
           d4 xacc[HT];
 #pragma unroll
           for (int ht = 0; ht < HT; ++ht) xacc[ht] = d4{0.0, 0.0, 0.0, 0.0};
-          for (int kp = 0; kp < KH.nprim; ++kp) {
-            const PrimRec PK = A.prim[KH.prim0 + kp];
+          for (int kp = 0; kp < CD.nprim; ++kp) {
+            const PrimRec PK = kp == 0 ? first_prim(CD) : A.prim[CD.prim0 + kp];
             if (!single) rtable<E + F>(s_R[0], s_R[1], PB, PK, A.kappa, A.boys, lane);
             const double *etk = A.et + PK.et;
             const double *Rf = s_R[0];
@@ -559,12 +563,11 @@ __global__ void __launch_bounds__(256) md_lane_kernel(MdArgs A) {
       jc = ir;
     }
     if (ir >= A.nrows) continue;
-    const PairDesc RD = A.rdesc[A.rows[ir]];
-    const PairDesc CD = A.cdesc[A.cols[jc]];
-    const PairHdr BH = A.hdr[RD.pt], KH = A.hdr[CD.pt];
-    const PrimRec PB = A.prim[BH.prim0], PK = A.prim[KH.prim0];
-    const int nr = pair_nrow(RD, BH), nc = pair_nrow(CD, KH);
-    const int ldb = BH.n0 * BH.n1, ldk = KH.n0 * KH.n1;
+    const QRec RD = A.rrec[ir];
+    const QRec CD = A.crec[jc];
+    const PrimRec PB = first_prim(RD), PK = first_prim(CD);
+    const int nr = RD.nrow, nc = CD.nrow;
+    const int ldb = RD.n0 * RD.n1, ldk = CD.n0 * CD.n1;
     const double *etb = A.et + PB.et, *etk = A.et + PK.et;
     double *dst = A.out + RD.off + CD.off;
     double R[nherm(L)];
@@ -572,7 +575,7 @@ __global__ void __launch_bounds__(256) md_lane_kernel(MdArgs A) {
 #pragma unroll 1
     for (int r = 0; r < NRM; ++r) {
       if (r < nr) {
-        const int fr = pair_full(RD, BH, A.sel, r);
+        const int fr = pair_full(RD, A.sel, r);
         double eb[NHE];
 #pragma unroll
         for (int h = 0; h < NHE; ++h) eb[h] = etb[(long long)h * ldb + fr];
@@ -595,7 +598,7 @@ __global__ void __launch_bounds__(256) md_lane_kernel(MdArgs A) {
 #pragma unroll
           for (int c = 0; c < NCM; ++c) {
             if (c < nc) {
-              const int fc = pair_full(CD, KH, A.sel, c);
+              const int fc = pair_full(CD, A.sel, c);
               double v = 0.0;
 #pragma unroll
               for (int k = 0; k < NHF; ++k) v += y[k] * etk[(long long)k * ldk + fc];
