@@ -73,6 +73,37 @@ def profiled_traffic():
     return None
 
 
+def batched_form(nworkers: int):
+    """The batched form on 64 independent molecules ((H2O)n, n uniform in
+    1..12, O 6s4p2d / H 3s1p): the same decompositions one at a time (plain)
+    and concurrently (gi_chol_build_batch), after one warm-up pass."""
+    from gigpu import lib, systems
+    mols = systems.batch_systems(64, seed=3)
+    ctxs = [lib.Context(0) for _ in mols]
+    try:
+        for c, b in zip(ctxs, mols):
+            c.set_basis(b)
+        lib.chol_build_batch(ctxs, tau=1e-6, nworkers=nworkers)
+        t0 = time.perf_counter()
+        for c in ctxs:
+            c.chol_build(tau=1e-6)
+        t_plain = time.perf_counter() - t0
+        nv_plain = [c.chol_dims()["nvecs"] for c in ctxs]
+        quartets = sum(c.chol_dims()["nquartets"] for c in ctxs)
+        t0 = time.perf_counter()
+        lib.chol_build_batch(ctxs, tau=1e-6, nworkers=nworkers)
+        t_batch = time.perf_counter() - t0
+        nv_batch = [c.chol_dims()["nvecs"] for c in ctxs]
+    finally:
+        for c in ctxs:
+            c.close()
+    return {"workload": "64 independent (H2O)n, n uniform in 1..12 (seed 3), O 6s4p2d / H 3s1p, tau=1e-6",
+            "nao_total": int(sum(b.nao for b in mols)), "nvecs_total": int(sum(nv_batch)),
+            "same_nvecs": nv_plain == nv_batch, "quartets": int(quartets), "workers": nworkers,
+            "plain_s": t_plain, "batched_s": t_batch, "speedup": t_plain / t_batch,
+            "plain_quartets_per_s": quartets / t_plain, "batched_quartets_per_s": quartets / t_batch}
+
+
 def cpu_baseline(seed: int):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
@@ -98,6 +129,8 @@ def main():
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mo", action="store_true")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched-form comparison")
+    ap.add_argument("--batch-workers", type=int, default=8)
     ap.add_argument("--eri-stats", action="store_true",
                     help="print per-class ERI kernel statistics to stderr (sets GIGPU_ERI_STATS)")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
@@ -189,7 +222,7 @@ def main():
         "data": "synthetic: seeded (H2O)%d cluster, even-tempered basis, random orthonormal MO coefficients" % args.nwater,
         "config": {"workload": "plain FP64 two-step Cholesky of (H2O)%d, O 8s5p3d1f / H 4s2p1d, tau=1e-6, "
                                "+ AO->MO transform nocc=40 nvirt=300" % args.nwater,
-                   "nao": basis.nao, "tau": 1e-6, "form": "plain FP64 GPU (batched/mixed forms not built)"},
+                   "nao": basis.nao, "tau": 1e-6, "form": "plain FP64 GPU (the batched form is reported under \"batched\"; mixed precision not built)"},
         "cholesky_build_s": t_build,
         "dims": {k: d[k] for k in ("nvecs", "norb_pairs", "nshell_pairs", "nsubsets", "max_subset_dim",
                                    "npivot_shell_pairs", "nmacro")},
@@ -211,17 +244,20 @@ def main():
                      "launches": launches, "flops_per_launch": eri_flops / max(launches, 1),
                      "flops": eri_flops, "kernel_s": eri_s},
     }
+    stat_rows = sorted(ctx.eri_class_stats(), key=lambda r: -r[3]) if args.eri_stats else []
+    ctx.close()
+    if not args.no_batched:
+        out["batched"] = batched_form(args.batch_workers)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.seed)
     if args.eri_stats:
-        rows = sorted(ctx.eri_class_stats(), key=lambda r: -r[3])
+        rows = stat_rows
         tot = sum(r[3] for r in rows)
         print("E F      quartets     ms    TFLOP/s  share", file=sys.stderr)
         for E, F, nq, ms, fl in rows:
             print("%d %d %13d %8.2f %8.3f %6.1f%%" % (E, F, nq, ms, fl / ms / 1e9 if ms else 0,
                                                     100 * ms / tot), file=sys.stderr)
     print(json.dumps(out))
-    ctx.close()
 
 
 if __name__ == "__main__":

@@ -1,7 +1,9 @@
 // C ABI of libgigpu (declarations and reference citations in include/gigpu.h).
 #include "../../include/gigpu.h"
+#include <atomic>
 #include <cstring>
 #include <memory>
+#include <thread>
 #include "gi_context.h"
 
 namespace gi {
@@ -33,6 +35,7 @@ using gi::Error;
 static gi::Context &ctx_of(gi_context *c) {
   if (!c || !c->c) throw Error{7, "null gi_context"};
   GI_HIP(hipSetDevice(c->c->device));
+  gi::g_alloc_stream = c->c->stream;
   return *c->c;
 }
 
@@ -212,6 +215,41 @@ int gi_set_option(gi_context *cp, const char *name, int64_t value) {
     else if (n == "eri_lane") C.eri_lane = value != 0;
     else if (n == "eri_trans") C.eri_trans = value != 0;
     else throw Error{9, "unknown option " + n};
+  })
+}
+
+int gi_chol_build_batch(int64_t n, gi_context *const *ctxs, double tau, double kappa, int64_t max_block_dim,
+                        int64_t vec_block, int64_t flags, int64_t nworkers, int64_t *status) {
+  GI_TRY({
+    if (n < 0) throw Error{8, "negative batch size"};
+    if (n == 0) return 0;
+    if (!ctxs || !status) throw Error{8, "null batch arrays"};
+    std::atomic<int64_t> next{0};
+    std::vector<std::string> msgs(n);
+    auto worker = [&]() {
+      for (;;) {
+        const int64_t i = next.fetch_add(1);
+        if (i >= n) break;
+        try {
+          gi::Context &C = ctx_of(ctxs[i]);
+          gi::chol_build(C, tau, kappa, max_block_dim, vec_block, flags);
+          status[i] = 0;
+        } catch (const gi::Error &e) {
+          status[i] = e.code ? e.code : 1;
+          msgs[i] = e.msg;
+        } catch (const std::exception &e) {
+          status[i] = 99;
+          msgs[i] = std::string("exception: ") + e.what();
+        }
+      }
+    };
+    const int64_t nw = std::max<int64_t>(1, std::min<int64_t>(nworkers, n));
+    std::vector<std::thread> pool;
+    for (int64_t w = 1; w < nw; ++w) pool.emplace_back(worker);
+    worker();
+    for (auto &t : pool) t.join();
+    for (int64_t i = 0; i < n; ++i)
+      if (status[i] != 0) throw Error{(int)status[i], "batch entry " + std::to_string(i) + ": " + msgs[i]};
   })
 }
 

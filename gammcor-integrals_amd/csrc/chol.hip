@@ -16,7 +16,6 @@
 // bookkeeping whose order fixes the pivot sequence: the shell-pair sorts
 // (Singleton quicksort, identical permutation for tied keys), the BASE
 // screening, the batch selection and the strict-> first-maximum rule.
-#include <rocsolver/rocsolver.h>
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -306,10 +305,72 @@ __global__ void k_blockmax(const double *__restrict__ D, long long n, double *__
   if (threadIdx.x == 0) part[blockIdx.x] = sv[0];
 }
 
+// grid (ceil(n / 256), min(n, 65535)), 256 lanes; columns grid-stride in y
 __global__ void k_zero_upper(double *A, long long n) {
-  long long c = blockIdx.y;
-  for (long long r = threadIdx.x + blockIdx.x * (long long)blockDim.x; r < c && r < n; r += (long long)gridDim.x * blockDim.x)
-    A[r + n * c] = 0.0;
+  for (long long c = blockIdx.y; c < n; c += gridDim.y)
+    for (long long r = threadIdx.x + blockIdx.x * (long long)blockDim.x; r < c && r < n;
+         r += (long long)gridDim.x * blockDim.x)
+      A[r + n * c] = 0.0;
+}
+
+// ---- V = L L^T and L^-1 of the pivot block (real_Cholesky, real_linalg.f90:39-57,
+// and real_LowerTriangularInverse, :60-72, called at TwoStepCholesky.f90:344-350).
+// LAPACK dpotrf/dtrtri semantics: lower triangle, info = the order of the
+// first leading minor that is not positive definite.  Right-looking blocked
+// Cholesky: a POTF_NB-wide diagonal block is factored in LDS by one workgroup
+// (k_potf2_block), the panel below it is solved with dtrsm and the trailing
+// matrix updated with one dgemm.  The inverse solves L X = I column block by
+// column block, each block only on the rows at and below its diagonal.
+constexpr int POTF_NB = 64;
+constexpr int TRTRI_W = 512;
+
+__global__ void k_set_int(int *p, int v) { *p = v; }
+
+// unblocked Cholesky of A(k0:k0+b, k0:k0+b), b <= POTF_NB; does nothing once
+// an earlier block has failed
+__global__ void __launch_bounds__(256) k_potf2_block(double *__restrict__ A, long long lda, long long k0, int b,
+                                                     int *__restrict__ info) {
+  __shared__ double s[POTF_NB][POTF_NB + 1];
+  __shared__ int s_info;
+  const int tid = threadIdx.x;
+  if (tid == 0) s_info = *info;
+  __syncthreads();
+  if (s_info != 0) return;
+  for (int e = tid; e < b * b; e += 256) {
+    const int r = e % b, c = e / b;
+    s[r][c] = r >= c ? A[(k0 + r) + lda * (k0 + c)] : 0.0;
+  }
+  __syncthreads();
+  int jend = b;
+  for (int j = 0; j < b; ++j) {
+    const double d = s[j][j];
+    if (!(d > 0.0)) {   // the same LDS value in every lane: a uniform exit
+      jend = j;
+      break;
+    }
+    const double sq = sqrt(d);
+    __syncthreads();   // every lane has read s[j][j]
+    for (int r = j + tid; r < b; r += 256) s[r][j] = r == j ? sq : s[r][j] / sq;
+    __syncthreads();
+    const int m = b - j - 1;
+    for (int e = tid; e < m * m; e += 256) {
+      const int rr = j + 1 + e % m, cc = j + 1 + e / m;
+      if (rr >= cc) s[rr][cc] -= s[rr][j] * s[cc][j];
+    }
+    __syncthreads();
+  }
+  for (int e = tid; e < b * b; e += 256) {
+    const int r = e % b, c = e / b;
+    if (r >= c) A[(k0 + r) + lda * (k0 + c)] = s[r][c];
+  }
+  if (tid == 0 && jend < b) *info = (int)(k0 + jend + 1);
+}
+
+// same grid as k_zero_upper
+__global__ void k_identity(double *X, long long n) {
+  const long long r = threadIdx.x + blockIdx.x * (long long)blockDim.x;
+  if (r >= n) return;
+  for (long long c = blockIdx.y; c < n; c += gridDim.y) X[r + n * c] = r == c ? 1.0 : 0.0;
 }
 
 // ------------------------------------------------------------------ helpers
@@ -342,6 +403,48 @@ struct EriPhaseTimer {
     C.chol.eri_quartets[name] += C.eri_quartets_acc - q0;
   }
 };
+
+// V (n x n, lower triangle significant) -> L^-1 in place; throws the
+// reference's errors when V is not positive definite
+static void factor_and_invert(Context &C, DBuf<double> &V, long long n) {
+  DBuf<int> dinfo;
+  dinfo.alloc(1);
+  hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, C.stream, dinfo.p, 0);
+  const double one = 1.0, mone = -1.0;
+  for (long long k0 = 0; k0 < n; k0 += POTF_NB) {
+    const int b = (int)std::min<long long>(POTF_NB, n - k0);
+    hipLaunchKernelGGL(k_potf2_block, dim3(1), dim3(256), 0, C.stream, V.p, n, k0, b, dinfo.p);
+    const long long m = n - k0 - b;
+    if (m > 0) {
+      double *Lkk = V.p + k0 + n * k0, *A21 = V.p + (k0 + b) + n * k0, *A22 = V.p + (k0 + b) + n * (k0 + b);
+      ROCBLAS_CHECK(rocblas_dtrsm(C.blas, rocblas_side_right, rocblas_fill_lower, rocblas_operation_transpose,
+                                  rocblas_diagonal_non_unit, m, b, &one, Lkk, n, A21, n));
+      ROCBLAS_CHECK(rocblas_dgemm(C.blas, rocblas_operation_none, rocblas_operation_transpose, m, m, b, &mone, A21,
+                                  n, A21, n, &one, A22, n));
+    }
+  }
+  int info = 0;
+  GI_HIP(hipMemcpyAsync(&info, dinfo.p, sizeof(info), hipMemcpyDeviceToHost, C.stream));
+  GI_HIP(hipStreamSynchronize(C.stream));
+  if (info != 0) throw Error{40, "Cholesky factorization failed with exit code info=" + std::to_string(info)};
+  const dim3 gsq((unsigned)((n + 255) / 256), (unsigned)std::min<long long>(n, 65535));
+  hipLaunchKernelGGL(k_zero_upper, gsq, dim3(256), 0, C.stream, V.p, n);
+  // X = L^-1: column block [c0, c0 + w) is zero above row c0 and solves
+  // L(c0:, c0:) X(c0:, c0:c0+w) = I(c0:, c0:c0+w)
+  DBuf<double> X;
+  X.alloc((size_t)n * n);
+  hipLaunchKernelGGL(k_identity, gsq, dim3(256), 0, C.stream, X.p, n);
+  for (long long c0 = 0; c0 < n; c0 += TRTRI_W) {
+    const long long w = std::min<long long>(TRTRI_W, n - c0);
+    ROCBLAS_CHECK(rocblas_dtrsm(C.blas, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none,
+                                rocblas_diagonal_non_unit, n - c0, w, &one, V.p + c0 + n * c0, n,
+                                X.p + c0 + n * c0, n));
+  }
+  std::swap(V.p, X.p);
+  std::swap(V.n, X.n);
+  std::swap(V.st, X.st);
+  GI_HIP(hipStreamSynchronize(C.stream));
+}
 
 // ------------------------------------------------------------------ driver
 void chol_build(Context &C, double tau, double kappa, long long max_block_dim, long long block_dim,
@@ -849,21 +952,7 @@ void chol_build(Context &C, double tau, double kappa, long long max_block_dim, l
   // V = L L^T, L^-1 (real_Cholesky + real_LowerTriangularInverse)
   {
     double tc = now();
-    DBuf<rocblas_int> dinfo;
-    dinfo.alloc(1);
-    ROCBLAS_CHECK(rocsolver_dpotrf(C.blas, rocblas_fill_lower, (rocblas_int)nvec, S.inv_l.p, (rocblas_int)nvec,
-                                   dinfo.p));
-    rocblas_int info = 0;
-    GI_HIP(hipMemcpyAsync(&info, dinfo.p, sizeof(info), hipMemcpyDeviceToHost, C.stream));
-    GI_HIP(hipStreamSynchronize(C.stream));
-    if (info != 0) throw Error{40, "Cholesky factorization failed with exit code info=" + std::to_string(info)};
-    hipLaunchKernelGGL(k_zero_upper, dim3((unsigned)((nvec + 255) / 256), (unsigned)nvec), dim3(256), 0, C.stream,
-                       S.inv_l.p, nvec);
-    ROCBLAS_CHECK(rocsolver_dtrtri(C.blas, rocblas_fill_lower, rocblas_diagonal_non_unit, (rocblas_int)nvec,
-                                   S.inv_l.p, (rocblas_int)nvec, dinfo.p));
-    GI_HIP(hipMemcpyAsync(&info, dinfo.p, sizeof(info), hipMemcpyDeviceToHost, C.stream));
-    GI_HIP(hipStreamSynchronize(C.stream));
-    if (info != 0) throw Error{41, "Lower triangular inverse failed with exit code info=" + std::to_string(info)};
+    factor_and_invert(C, S.inv_l, nvec);
     S.timings["potrf_trtri"] = now() - tc;
   }
   S.timings["step2"] = now() - t2;

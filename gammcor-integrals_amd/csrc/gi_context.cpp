@@ -5,6 +5,7 @@
 namespace gi {
 
 static thread_local std::string g_last_error;
+thread_local hipStream_t g_alloc_stream = nullptr;
 void set_error(const std::string &m) { g_last_error = m; }
 const char *last_error() { return g_last_error.c_str(); }
 
@@ -14,6 +15,14 @@ Context::Context(int dev) : device(dev) {
   if (dev < 0 || dev >= n) throw Error{1, "invalid HIP device index " + std::to_string(dev)};
   GI_HIP(hipSetDevice(dev));
   GI_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  g_alloc_stream = stream;
+  // keep freed stream-ordered memory in the device pool (reused by the next
+  // build instead of going back to the driver at every synchronization)
+  hipMemPool_t pool = nullptr;
+  if (hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess && pool) {
+    uint64_t keep = UINT64_MAX;
+    (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+  }
   if (rocblas_create_handle(&blas) != rocblas_status_success) throw Error{2, "rocblas_create_handle failed"};
   rocblas_set_stream(blas, stream);
   upload_constants(*this);
@@ -39,6 +48,12 @@ Context::~Context() {
   md_rec.release();
   chol.inv_l.release();
   chol.rkpq.release();
+  chol.mo_adj_off.release();
+  chol.mo_adj_col.release();
+  chol.mo_adj_p.release();
+  // every buffer is released (stream-ordered frees on `stream`) before the
+  // stream goes away
+  if (stream) (void)hipStreamSynchronize(stream);
   if (stream) (void)hipStreamDestroy(stream);
 }
 

@@ -13,17 +13,24 @@
 
 namespace gi {
 
+// Stream of the context the calling thread works for (set on every entry
+// through the C ABI).  Device buffers are allocated and freed stream-ordered
+// on it (hipMallocAsync / hipFreeAsync), so contexts driven from different
+// host threads never synchronize each other through allocation.
+extern thread_local hipStream_t g_alloc_stream;
+
 // owning device buffer
 template <class T>
 struct DBuf {
   T *p = nullptr;
   size_t n = 0;
+  hipStream_t st = nullptr;   // stream the buffer was allocated on
   DBuf() = default;
   DBuf(const DBuf &) = delete;
   DBuf &operator=(const DBuf &) = delete;
   ~DBuf() { release(); }
   void release() {
-    if (p) (void)hipFree(p);
+    if (p) (void)hipFreeAsync(p, st);
     p = nullptr;
     n = 0;
   }
@@ -31,7 +38,8 @@ struct DBuf {
     if (count <= n && p) return;
     release();
     if (count == 0) count = 1;
-    GI_HIP(hipMalloc((void **)&p, count * sizeof(T)));
+    st = g_alloc_stream;
+    GI_HIP(hipMallocAsync((void **)&p, count * sizeof(T), st));
     n = count;
   }
   void upload(const T *src, size_t count, hipStream_t s) {
@@ -44,12 +52,12 @@ struct DBuf {
     if (count <= n && p) return;
     size_t nn = std::max(count, n + n / 2);
     T *q = nullptr;
-    GI_HIP(hipMalloc((void **)&q, nn * sizeof(T)));
+    GI_HIP(hipMallocAsync((void **)&q, nn * sizeof(T), s));
     if (p && n) GI_HIP(hipMemcpyAsync(q, p, n * sizeof(T), hipMemcpyDeviceToDevice, s));
-    GI_HIP(hipStreamSynchronize(s));
     release();
     p = q;
     n = nn;
+    st = s;
   }
 };
 

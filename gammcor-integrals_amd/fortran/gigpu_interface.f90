@@ -75,6 +75,14 @@ module gigpu_c
       real(c_double), value :: tau, kappa
       integer(c_int64_t), value :: max_block_dim, vec_block, flags
     end function
+    integer(c_int) function gi_chol_build_batch(n, ctxs, tau, kappa, max_block_dim, vec_block, flags, nworkers, &
+        status) bind(C, name="gi_chol_build_batch")
+      import :: c_int, c_ptr, c_int64_t, c_double
+      integer(c_int64_t), value :: n, max_block_dim, vec_block, flags, nworkers
+      type(c_ptr), intent(in) :: ctxs(*)
+      real(c_double), value :: tau, kappa
+      integer(c_int64_t), intent(out) :: status(*)
+    end function
     integer(c_int) function gi_chol_accuracy_tau(accuracy, tau) bind(C, name="gi_chol_accuracy_tau")
       import :: c_int, c_int64_t, c_double
       integer(c_int64_t), value :: accuracy

@@ -17,7 +17,7 @@ EXPORTS = ["gi_create", "gi_destroy", "gi_last_error", "gi_set_basis", "gi_eri_q
            "gi_chol_build", "gi_chol_accuracy_tau", "gi_chol_dims", "gi_chol_meta", "gi_chol_rkpq",
            "gi_chol_rkpq_device", "gi_chol_inv_l", "gi_chol_vdiag", "gi_chol_timings",
            "gi_mo_transform", "gi_mo_transform_device", "gi_eri_class_stats", "gi_eri_pair_block",
-           "gi_set_option"]
+           "gi_set_option", "gi_chol_build_batch"]
 
 CHOL_DECODE_FIXED = 1
 CHOL_SKIP_VECTORS = 2
@@ -61,6 +61,8 @@ def load():
     L.gi_eri_class_stats.argtypes = [C.c_void_p, i64, dptr, iptr]
     L.gi_eri_pair_block.argtypes = [C.c_void_p, i64, iptr, i64, iptr, C.c_double, dptr, i64]
     L.gi_set_option.argtypes = [C.c_void_p, C.c_char_p, i64]
+    L.gi_chol_build_batch.argtypes = [i64, C.POINTER(C.c_void_p), C.c_double, C.c_double, i64, i64, i64, i64,
+                                      iptr]
     L.gi_mo_transform.argtypes = [C.c_void_p, dptr, i64, i64, i64, dptr, i64, i64, i64, i64, i64, dptr]
     L.gi_mo_transform_device.argtypes = [C.c_void_p, C.c_void_p, i64, i64, i64, C.c_void_p, i64, i64, i64, i64,
                                          i64, C.c_void_p]
@@ -234,3 +236,15 @@ def accuracy_tau(accuracy: int) -> float:
     t = C.c_double()
     _check(L.gi_chol_accuracy_tau(accuracy, C.byref(t)))
     return t.value
+
+
+def chol_build_batch(contexts, tau=1e-6, kappa=0.0, max_block_dim=4000, vec_block=500, flags=0, nworkers=8):
+    """Decompose the systems of several contexts (each with its basis set)
+    concurrently: gi_chol_build_batch.  Returns the per-context statuses."""
+    L = load()
+    n = len(contexts)
+    handles = (C.c_void_p * max(n, 1))(*[c.h.value for c in contexts])
+    status = np.zeros(max(n, 1), np.int64)
+    _check(L.gi_chol_build_batch(n, handles, tau, kappa, max_block_dim, vec_block, flags, nworkers,
+                                 _p(status, i64)))
+    return status[:n]

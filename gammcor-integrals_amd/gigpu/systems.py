@@ -90,6 +90,18 @@ def config_large_cluster(seed: int = 2) -> AOBasis:
     return water_system(16, 10.0, "large", seed)
 
 
+def batch_systems(nmol: int = 64, seed: int = 3, nmin: int = 1, nmax: int = 12,
+                  basis: str = "small") -> list:
+    """Independent molecules for the batched form: (H2O)n with n uniform in
+    nmin..nmax (seeded), each in a cube of 6 A * (n / 4)^(1/3) (the 6 A box
+    of the (H2O)4 workload at the same water density), each with its own
+    seeded geometry and the same basis specification."""
+    rng = np.random.default_rng(seed)
+    ns = rng.integers(nmin, nmax + 1, nmol)
+    return [water_system(int(n), 6.0 * (n / 4.0) ** (1.0 / 3.0), basis, seed=seed * 1000 + k)
+            for k, n in enumerate(ns)]
+
+
 def write_xyz(path: str, symbols, coords_angstrom) -> None:
     """XYZ block as read by sys_Read_XYZ (sys_definitions.f90:588-723)."""
     with open(path, "w") as f:

@@ -102,6 +102,15 @@ int gi_set_option(gi_context *ctx, const char *name, int64_t value);
  * (TwoStepCholesky_definitions.f90:20-53). */
 int gi_chol_build(gi_context *ctx, double tau, double kappa, int64_t max_block_dim,
                   int64_t vec_block, int64_t flags);
+/* Batched form (no single reference counterpart; the reference decomposes one
+ * system per chol_gammcor_Rkpq call): the decompositions of n independent
+ * systems, one per context (each with its basis set), run concurrently from
+ * `nworkers` host threads, each context on its own stream, so that small
+ * systems overlap on the GPU.  Same arguments and results per context as
+ * gi_chol_build; status[i] receives each entry's status, and the call fails
+ * (naming the first failed entry) if any entry failed. */
+int gi_chol_build_batch(int64_t n, gi_context *const *ctxs, double tau, double kappa, int64_t max_block_dim,
+                        int64_t vec_block, int64_t flags, int64_t nworkers, int64_t *status);
 /* tau of a CHOL_ACCURACY_* level; returns 0 and sets *tau, or fails with
  * "Invalid accuracy setting" (Cholesky_Gammcor.f90:407-409). */
 int gi_chol_accuracy_tau(int64_t accuracy, double *tau);

@@ -106,3 +106,30 @@ def test_gammcor_interface_and_mo(gpu_ctx, oracle_mod):
     assert np.abs(Rab2 - Rref2).max() <= 1e-10 * max(1.0, np.abs(Rref2).max())
     with pytest.raises(gammcor.GammcorError, match="Buffer size too small"):
         gammcor.chol_gammcor_Rkab(None, C, 1, nocc, C, 1, 5, 0, vecs, b, gammcor.ORBITAL_ORDERING_DALTON)
+
+
+def test_range_separated_cholesky(gpu_ctx, oracle_mod):
+    """Omega > 0 (Kappa = 1/Omega^2, Cholesky_Gammcor.f90:411-415): the
+    long-range Coulomb matrix decomposed with the same pivots as the oracle."""
+    b = systems.water_system(1, 6.0, "small", seed=13)
+    o = oracle_mod.Oracle(b)
+    omega = 0.7
+    kappa = 1.0 / omega ** 2
+    ref = o.chol(tau=1e-6, kappa=kappa)
+    assert ref.status == 0, ref.message
+    gpu_ctx.set_basis(b)
+    gpu_ctx.chol_build(tau=1e-6, kappa=kappa)
+    m = gpu_ctx.chol_meta()
+    assert m["nvecs"] == ref.nvecs
+    assert np.array_equal(m["pivot_order"], ref.pivot_order)
+    assert np.abs(gpu_ctx.chol_vdiag() - ref.vdiag).max() <= 1e-12
+    Rg = full_vectors(m["shell_pair_loc"], m["shell_pair_dim"], gpu_ctx.chol_rkpq(), m["norb_pairs"])
+    Rr = ref.full_vectors()
+    P = step2_columns(b, m)
+    Gg, Gr = Rg.T @ Rg[:, P], Rr.T @ Rr[:, P]
+    assert np.abs(Gg - Gr).max() <= 1e-11 * np.abs(Gr).max()
+    # the attenuated operator has fewer significant vectors than the full one
+    full = o.chol(tau=1e-6)
+    assert ref.nvecs < full.nvecs
+    vecs = gammcor.chol_gammcor_Rkpq(b, gammcor.CHOL_ACCURACY_TIGHT, Omega=omega, ctx=gpu_ctx)
+    assert vecs.Chol2Data.NVecs == ref.nvecs
