@@ -48,3 +48,37 @@ def test_batch_reports_failing_entry():
     finally:
         for c in ctxs:
             c.close()
+
+
+def _fingerprint(c):
+    m = c.chol_meta()
+    x = c.chol_inv_l()
+    return m["nvecs"], m["pivot_order"].copy(), np.diag(x).copy(), float(np.abs(x).sum())
+
+
+def test_full_batch_repeated_under_load():
+    """32 systems on 8 workers, twice: every entry succeeds and matches its
+    one-at-a-time build (pivot order exactly, L^-1 to rounding).  The
+    factorization of the pivot block runs while other entries' ERI, GEMM and
+    pivot-search kernels share the GPU."""
+    mols = systems.batch_systems(nmol=32, seed=3, nmin=1, nmax=8)
+    ctxs = [lib.Context(0) for _ in mols]
+    try:
+        for c, b in zip(ctxs, mols):
+            c.set_basis(b)
+        ref = []
+        for c in ctxs:
+            c.chol_build(tau=1e-6)
+            ref.append(_fingerprint(c))
+        for rep in range(2):
+            status = lib.chol_build_batch(ctxs, tau=1e-6, nworkers=8)
+            assert list(status) == [0] * len(mols)
+            for i, (c, (nv, piv, dg, s)) in enumerate(zip(ctxs, ref)):
+                nv2, piv2, dg2, s2 = _fingerprint(c)
+                assert nv2 == nv, (rep, i)
+                assert np.array_equal(piv2, piv), (rep, i)
+                assert np.abs(dg2 - dg).max() <= 1e-10 * np.abs(dg).max(), (rep, i)
+                assert abs(s2 - s) <= 1e-10 * s, (rep, i)
+    finally:
+        for c in ctxs:
+            c.close()
