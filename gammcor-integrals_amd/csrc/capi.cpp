@@ -87,6 +87,7 @@ int gi_set_basis(gi_context *cp, int64_t natoms, const double *coords, int64_t n
     C.shell_nang.assign(nshells, 0);
     C.shell_np.assign(nshells, 0);
     C.shell_loc.assign(nshells, 0);
+    C.shell_atom.assign(nshells, 0);
     std::vector<double> hexp, hcn, hnorm;
     int loc = 0;
     for (int64_t s = 0; s < nshells; ++s) {
@@ -115,6 +116,7 @@ int gi_set_basis(gi_context *cp, int64_t natoms, const double *coords, int64_t n
       C.shell_nang[s] = nang;
       C.shell_np[s] = np;
       C.shell_loc[s] = loc;
+      C.shell_atom[s] = (int)a;
       loc += nang;
     }
     C.nao = loc;
@@ -250,6 +252,14 @@ int gi_chol_build_batch(int64_t n, gi_context *const *ctxs, double tau, double k
     for (auto &t : pool) t.join();
     for (int64_t i = 0; i < n; ++i)
       if (status[i] != 0) throw Error{(int)status[i], "batch entry " + std::to_string(i) + ": " + msgs[i]};
+  })
+}
+
+int gi_coulomb_energy_grad(gi_context *cp, const double *density, int64_t ldd, double kappa, double *energy,
+                           double *grad) {
+  GI_TRY({
+    gi::Context &C = ctx_of(cp);
+    gi::coulomb_energy_grad(C, density, ldd, kappa, energy, grad);
   })
 }
 

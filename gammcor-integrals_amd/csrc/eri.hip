@@ -10,6 +10,7 @@
 #include <map>
 #include <unordered_map>
 #include "gi_context.h"
+#include "grad_kernels.h"
 
 namespace gi {
 
@@ -159,6 +160,13 @@ static void init_md_table() {
 #undef GI_MD
 #undef GI_LANE
   done = true;
+}
+
+void launch_md_class(int E, int F, const MdArgs &a, int grid, hipStream_t s) {
+  init_md_table();
+  if (E < 0 || E > MD_EMAX || F < 0 || F > MD_EMAX || !md_table[E][F])
+    throw Error{60, "no ERI kernel for Hermite orders " + std::to_string(E) + ", " + std::to_string(F)};
+  md_table[E][F](a, grid, s);
 }
 
 void upload_constants(Context &C) {

@@ -94,6 +94,7 @@ struct Context {
   int natoms = 0, nshells = 0, spher = 1, lmax = 0, nao = 0;
   std::vector<DevShell> hshells;
   std::vector<int> shell_l, shell_nang, shell_np, shell_loc;   // shell_loc: 0-based first AO
+  std::vector<int> shell_atom;                                  // 0-based atom of each shell
   DBuf<DevShell> dshells;
   DBuf<double> dexps, dcntr, dnorms, dboys;
   bool basis_set = false;
@@ -151,5 +152,23 @@ void eri_run(Context &C, const EriRequest &R);
 
 // one-time device tables
 void upload_constants(Context &C);
+
+// one md_kernel<E, F> launch (eri.hip)
+void launch_md_class(int E, int F, const MdArgs &a, int grid, hipStream_t s);
+
+// density-contracted pair tables of the Coulomb gradient (grad_kernels.h, eri.hip)
+constexpr int GRAD_NCOL = 10;   // columns per primitive pair table
+struct GradJob {
+  int s0, s1, ia, ib;   // shells (s0 >= s1) and primitive indices
+  int prim, pad;        // PrimRec slot
+  long long et;         // table offset (doubles)
+  double w;             // density weight: 2 for two different shells, else 1
+};
+void launch_build_grad_et(const Context &C, const int *ao_loc, const double *D, long long ldd,
+                          const GradJob *jobs, long long njobs, PrimRec *prims, double *et);
+
+// Coulomb energy of a density and its nuclear gradient (grad.hip)
+void coulomb_energy_grad(Context &C, const double *D, long long ldd, double kappa, double *energy,
+                         double *grad);
 
 }  // namespace gi

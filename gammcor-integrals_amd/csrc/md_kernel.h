@@ -150,6 +150,8 @@ struct MdArgs {
   long long nrows, ncols, ntiles;
   int mode, kt;                  // LAUNCH_*, ket pairs per tile (PRODUCT)
   int trans;                     // kernel bra side = caller's ket side (store transposed)
+  int rowsum;                    // add each row's sum over the ket columns to out[off + r]
+                                 // (atomically; PRODUCT, not transposed)
   double kappa;
   double *out;
   long long ldo;
@@ -329,6 +331,22 @@ __global__ void __launch_bounds__(64) md_kernel(MdArgs A) {
           }
         }
         // lane (j, g) holds V(row 16 rt + j, queued column 4 qq + g)
+        if (A.rowsum) {
+#pragma unroll
+          for (int rt = 0; rt < RT; ++rt) {
+            double sv = 0.0;
+#pragma unroll
+            for (int qq = 0; qq < 4; ++qq) sv += (4 * qq + g < n) ? vacc[rt][qq] : 0.0;
+            sv += __shfl_xor(sv, 16);
+            sv += __shfl_xor(sv, 32);
+            const int r = 16 * rt + i16;
+            if (g == 0 && rt < nrt && r < nr) atomicAdd(out + r, sv);
+          }
+          head = (head + n) & (MD_RING - 1);
+          q -= n;
+          __syncthreads();   // ring slots are free again
+          return;
+        }
 #pragma unroll
         for (int qq = 0; qq < 4; ++qq) {
           const int ci = 4 * qq + g;

@@ -83,6 +83,16 @@ module gigpu_c
       real(c_double), value :: tau, kappa
       integer(c_int64_t), intent(out) :: status(*)
     end function
+    integer(c_int) function gi_coulomb_energy_grad(ctx, density, ldd, kappa, energy, grad) &
+        bind(C, name="gi_coulomb_energy_grad")
+      import :: c_int, c_ptr, c_int64_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int64_t), value :: ldd
+      real(c_double), intent(in) :: density(ldd, *)
+      real(c_double), value :: kappa
+      real(c_double), intent(out) :: energy
+      type(c_ptr), value :: grad
+    end function
     integer(c_int) function gi_chol_accuracy_tau(accuracy, tau) bind(C, name="gi_chol_accuracy_tau")
       import :: c_int, c_int64_t, c_double
       integer(c_int64_t), value :: accuracy

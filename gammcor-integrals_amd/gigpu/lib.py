@@ -17,7 +17,7 @@ EXPORTS = ["gi_create", "gi_destroy", "gi_last_error", "gi_set_basis", "gi_eri_q
            "gi_chol_build", "gi_chol_accuracy_tau", "gi_chol_dims", "gi_chol_meta", "gi_chol_rkpq",
            "gi_chol_rkpq_device", "gi_chol_inv_l", "gi_chol_vdiag", "gi_chol_timings",
            "gi_mo_transform", "gi_mo_transform_device", "gi_eri_class_stats", "gi_eri_pair_block",
-           "gi_set_option", "gi_chol_build_batch"]
+           "gi_set_option", "gi_chol_build_batch", "gi_coulomb_energy_grad"]
 
 CHOL_DECODE_FIXED = 1
 CHOL_SKIP_VECTORS = 2
@@ -63,6 +63,7 @@ def load():
     L.gi_set_option.argtypes = [C.c_void_p, C.c_char_p, i64]
     L.gi_chol_build_batch.argtypes = [i64, C.POINTER(C.c_void_p), C.c_double, C.c_double, i64, i64, i64, i64,
                                       iptr]
+    L.gi_coulomb_energy_grad.argtypes = [C.c_void_p, dptr, i64, C.c_double, dptr, C.c_void_p]
     L.gi_mo_transform.argtypes = [C.c_void_p, dptr, i64, i64, i64, dptr, i64, i64, i64, i64, i64, dptr]
     L.gi_mo_transform_device.argtypes = [C.c_void_p, C.c_void_p, i64, i64, i64, C.c_void_p, i64, i64, i64, i64,
                                          i64, C.c_void_p]
@@ -194,6 +195,22 @@ class Context:
         out = np.zeros(n * n)
         _check(self.L.gi_chol_inv_l(self.h, _p(out)))
         return out.reshape(n, n).T
+
+    def coulomb_energy(self, density, kappa=0.0, grad=True):
+        """E_J = 1/2 sum D_pq (pq|rs) D_rs of a symmetric AO density (nao x nao),
+        and with grad=True also dE_J/dR as an (natoms, 3) array (Eh/bohr):
+        gi_coulomb_energy_grad.  Returns E or (E, G)."""
+        n = self.basis.nao
+        D = np.asfortranarray(np.asarray(density, dtype=np.float64))
+        if D.shape != (n, n):
+            raise ValueError("density must be nao x nao (%d)" % n)
+        e = np.zeros(1)
+        if grad:
+            g = np.zeros((self.basis.natoms, 3))
+            _check(self.L.gi_coulomb_energy_grad(self.h, _p(D), n, float(kappa), _p(e), g.ctypes.data))
+            return float(e[0]), g
+        _check(self.L.gi_coulomb_energy_grad(self.h, _p(D), n, float(kappa), _p(e), None))
+        return float(e[0])
 
     def chol_vdiag(self):
         n = self.basis.nao

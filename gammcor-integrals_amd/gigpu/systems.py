@@ -102,6 +102,21 @@ def batch_systems(nmol: int = 64, seed: int = 3, nmin: int = 1, nmax: int = 12,
             for k, n in enumerate(ns)]
 
 
+def config_gradient_cluster(seed: int = 5) -> AOBasis:
+    """(H2O)8 in a 6 A * 2^(1/3) box (the water density of the (H2O)4
+    workload), O 6s4p2d / H 3s1p: 320 spherical AOs, 24 atoms."""
+    return water_system(8, 6.0 * 2.0 ** (1.0 / 3.0), "small", seed)
+
+
+def occupied_density(nao: int, nocc: int, seed: int) -> np.ndarray:
+    """D = C_occ C_occ^T from a random orthonormal C (QR of an N(0, 1)
+    nao x nao matrix, seeded), C_occ its first nocc columns."""
+    rng = np.random.default_rng(seed)
+    q, _ = np.linalg.qr(rng.standard_normal((nao, nao)))
+    c = q[:, :nocc]
+    return c @ c.T
+
+
 def write_xyz(path: str, symbols, coords_angstrom) -> None:
     """XYZ block as read by sys_Read_XYZ (sys_definitions.f90:588-723)."""
     with open(path, "w") as f:

@@ -111,6 +111,16 @@ int gi_chol_build(gi_context *ctx, double tau, double kappa, int64_t max_block_d
  * (naming the first failed entry) if any entry failed. */
 int gi_chol_build_batch(int64_t n, gi_context *const *ctxs, double tau, double kappa, int64_t max_block_dim,
                         int64_t vec_block, int64_t flags, int64_t nworkers, int64_t *status);
+/* Differentiation form (no reference counterpart: the reference has no
+ * gradient code).  The Coulomb energy of a symmetric AO density,
+ *   E_J = 1/2 sum_{pqrs} D_pq (pq|rs) D_rs,
+ * D column-major (nao x nao, leading dimension ldd >= nao, the AO order of
+ * gi_set_basis), kappa as in gi_chol_build (0: plain Coulomb), and, when grad
+ * is not NULL, dE_J/dR for every atom (3 x natoms, x fastest, Eh/bohr) from
+ * derivative integrals evaluated on the fly.  The gradient needs l0 + l1 <= 5
+ * for every shell pair. */
+int gi_coulomb_energy_grad(gi_context *ctx, const double *density, int64_t ldd, double kappa, double *energy,
+                           double *grad);
 /* tau of a CHOL_ACCURACY_* level; returns 0 and sets *tau, or fails with
  * "Invalid accuracy setting" (Cholesky_Gammcor.f90:407-409). */
 int gi_chol_accuracy_tau(int64_t accuracy, double *tau);

@@ -119,6 +119,10 @@ int64_t or_mo_transform(const or_basis *b, const or_chol_result *res, int64_t na
                         const double *CB, int64_t ldb, int64_t b0, int64_t b1,
                         int64_t max_buffer_mb, double *Rkab);
 
+/* ---- Coulomb energy of an AO density (coulomb.c) ---- */
+/* E_J = 1/2 sum D_pq (pq|rs) D_rs; D column-major (nao x nao, leading dim ldd). */
+double or_coulomb_energy(const or_basis *b, const double *D, int64_t ldd, double kappa);
+
 /* ---- dense linear algebra used by the oracle (LAPACK/BLAS semantics of
  *      dpotrf "L", dtrtri "L","N" and dgemm, real_linalg.f90:39-72,1079-1361) ---- */
 int64_t or_dpotrf_lower(int64_t n, double *A, int64_t lda);

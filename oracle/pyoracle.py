@@ -73,6 +73,8 @@ def lib():
         L.or_nang.restype = i64
         L.or_eri_quartet.argtypes = [C.POINTER(OrBasis), i64, i64, i64, i64, C.c_double, dptr]
         L.or_eri_quartet_single_boys.argtypes = [C.POINTER(OrBasis), i64, i64, i64, i64, C.c_double, dptr]
+        L.or_coulomb_energy.argtypes = [C.POINTER(OrBasis), dptr, i64, C.c_double]
+        L.or_coulomb_energy.restype = C.c_double
         L.or_dsort0.argtypes = [dptr, iptr, i64, i64]
         L.or_chol_build.argtypes = [C.POINTER(OrBasis), C.c_double, C.c_double, i64, i64, i64,
                                     C.POINTER(OrCholResult)]
@@ -117,6 +119,11 @@ class Oracle:
         f = self.L.or_eri_quartet_single_boys if single_boys else self.L.or_eri_quartet
         f(C.byref(self.ob), sa, sb, sc, sd, kappa, _p(V))
         return V.reshape(n[3], n[2], n[1], n[0]).transpose(3, 2, 1, 0)
+
+    def coulomb_energy(self, density, kappa=0.0):
+        """E_J = 1/2 sum D_pq (pq|rs) D_rs of a symmetric AO density."""
+        D = np.asfortranarray(np.asarray(density, dtype=np.float64))
+        return float(self.L.or_coulomb_energy(C.byref(self.ob), _p(D), D.shape[0], kappa))
 
     def chol(self, tau=1e-6, kappa=0.0, max_block_dim=4000, vec_block=500, flags=0):
         res = OrCholResult()
