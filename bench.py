@@ -104,6 +104,39 @@ def batched_form(nworkers: int):
             "plain_quartets_per_s": quartets / t_plain, "batched_quartets_per_s": quartets / t_batch}
 
 
+def gradient_form(reps: int = 3):
+    """The differentiation form on (H2O)8, O 6s4p2d / H 3s1p (320 AOs, 72
+    coordinates), D = C_occ C_occ^T with nocc = 40: median seconds of the
+    energy-only call and of the energy + nuclear gradient call, beside what
+    central differences over all coordinates would cost (2 energies each)."""
+    from gigpu import lib, systems
+    basis = systems.config_gradient_cluster()
+    D = systems.occupied_density(basis.nao, 40, seed=7)
+    ctx = lib.Context(0)
+    try:
+        ctx.set_basis(basis)
+        ctx.coulomb_energy(D)
+        te, tg = [], []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            e = ctx.coulomb_energy(D, grad=False)
+            te.append(time.perf_counter() - t0)
+            t0 = time.perf_counter()
+            e2, g = ctx.coulomb_energy(D)
+            tg.append(time.perf_counter() - t0)
+    finally:
+        ctx.close()
+    nsh = basis.nshells
+    npair = nsh * (nsh + 1) // 2
+    t_e, t_g = float(np.median(te)), float(np.median(tg))
+    return {"workload": "E_J = 1/2 sum D_pq (pq|rs) D_rs and dE_J/dR of (H2O)8 (seed 5), O 6s4p2d / H 3s1p, "
+                        "D = C_occ C_occ^T (nocc=40, seed 7)",
+            "nao": basis.nao, "natoms": basis.natoms, "shell_pairs": npair,
+            "pair_products": npair * npair, "energy": e, "energy_s": t_e, "gradient_s": t_g,
+            "gradient_over_energy": t_g / t_e, "central_differences_s": 2 * 3 * basis.natoms * t_e,
+            "max_abs_gradient": float(np.abs(g).max())}
+
+
 def cpu_baseline(seed: int):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
@@ -131,6 +164,7 @@ def main():
     ap.add_argument("--no-mo", action="store_true")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-form comparison")
     ap.add_argument("--batch-workers", type=int, default=8)
+    ap.add_argument("--no-gradient", action="store_true", help="skip the differentiation-form timing")
     ap.add_argument("--eri-stats", action="store_true",
                     help="print per-class ERI kernel statistics to stderr (sets GIGPU_ERI_STATS)")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
@@ -248,6 +282,8 @@ def main():
     ctx.close()
     if not args.no_batched:
         out["batched"] = batched_form(args.batch_workers)
+    if not args.no_gradient:
+        out["gradient"] = gradient_form()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.seed)
     if args.eri_stats:
