@@ -104,6 +104,58 @@ def batched_form(nworkers: int):
             "plain_quartets_per_s": quartets / t_plain, "batched_quartets_per_s": quartets / t_batch}
 
 
+def precision_form(ctx, basis, Cmo, hip, reps: int = 3):
+    """Split-BF16 products with FP32 accumulation against the FP64 path on the
+    same (H2O)16 system (1200 AOs, the nbf of the precision configuration):
+    the AO->MO transform with nocc=80, nvirt=600 (device timings, max error
+    over all Rkab elements) and the Step-1 trailing update (build time,
+    vectors, and the reconstruction error max_pq (pq|pq) - sum_k R_k(pq)^2,
+    both with the Step-1 pivot decode so that the FP64 figure is the bound
+    tau)."""
+    from gigpu import lib
+    out = {}
+    ctx.set_basis(basis)
+    ctx.chol_build(tau=1e-6)
+    ref_res = ctx.chol_residual()[0]
+    nocc, nvirt = 80, 600
+    nv = ctx.chol_dims()["nvecs"]
+    dC = hip.alloc(Cmo.nbytes)
+    hip.h2d(dC, Cmo)
+    dR = hip.alloc(8 * nv * nocc * nvirt)
+    times = {}
+    for name, fl in (("fp64", 0), ("split_bf16", lib.MO_SPLIT_BF16)):
+        ctx.mo_transform_device(dC, basis.nao, 1, nocc, dC, basis.nao, nocc + 1, nocc + nvirt, dR, flags=fl)
+        t = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            ctx.mo_transform_device(dC, basis.nao, 1, nocc, dC, basis.nao, nocc + 1, nocc + nvirt, dR, flags=fl)
+            t.append(time.perf_counter() - t0)
+        times[name] = float(np.median(t))
+    hip.free(dR)
+    hip.free(dC)
+    r64 = ctx.mo_transform(Cmo, 1, nocc, Cmo, nocc + 1, nocc + nvirt)
+    r16 = ctx.mo_transform(Cmo, 1, nocc, Cmo, nocc + 1, nocc + nvirt, flags=lib.MO_SPLIT_BF16)
+    err = float(np.abs(r16 - r64).max())
+    rmax = float(np.abs(r64).max())
+    del r16, r64
+    out["mo_transform"] = {"nocc": nocc, "nvirt": nvirt, "nvecs": nv, "fp64_s": times["fp64"],
+                           "split_bf16_s": times["split_bf16"], "speedup": times["fp64"] / times["split_bf16"],
+                           "max_abs_err": err, "max_rel_err": err / rmax}
+    tu = {}
+    for name, fl in (("fp64", lib.CHOL_DECODE_FIXED), ("split_bf16", lib.CHOL_DECODE_FIXED | lib.CHOL_SPLIT_BF16)):
+        ctx.chol_build(tau=1e-6, flags=fl)
+        t0 = time.perf_counter()
+        ctx.chol_build(tau=1e-6, flags=fl)
+        t_b = time.perf_counter() - t0
+        tm = ctx.chol_timings()
+        mx, mn = ctx.chol_residual()
+        tu[name] = {"build_s": t_b, "subtract_r_s": tm["rtr"], "nvecs": ctx.chol_dims()["nvecs"],
+                    "residual_max": mx, "residual_min": mn}
+    out["trailing_update"] = tu
+    out["residual_max_reference_decode"] = ref_res
+    return out
+
+
 def gradient_form(reps: int = 3):
     """The differentiation form on (H2O)8, O 6s4p2d / H 3s1p (320 AOs, 72
     coordinates), D = C_occ C_occ^T with nocc = 40: median seconds of the
@@ -165,6 +217,7 @@ def main():
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-form comparison")
     ap.add_argument("--batch-workers", type=int, default=8)
     ap.add_argument("--no-gradient", action="store_true", help="skip the differentiation-form timing")
+    ap.add_argument("--no-precision", action="store_true", help="skip the split-BF16 precision-form comparison")
     ap.add_argument("--eri-stats", action="store_true",
                     help="print per-class ERI kernel statistics to stderr (sets GIGPU_ERI_STATS)")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
@@ -256,7 +309,7 @@ def main():
         "data": "synthetic: seeded (H2O)%d cluster, even-tempered basis, random orthonormal MO coefficients" % args.nwater,
         "config": {"workload": "plain FP64 two-step Cholesky of (H2O)%d, O 8s5p3d1f / H 4s2p1d, tau=1e-6, "
                                "+ AO->MO transform nocc=40 nvirt=300" % args.nwater,
-                   "nao": basis.nao, "tau": 1e-6, "form": "plain FP64 GPU (the batched form is reported under \"batched\"; mixed precision not built)"},
+                   "nao": basis.nao, "tau": 1e-6, "form": "plain FP64 GPU (the batched, precision and differentiation forms are reported under \"batched\", \"precision\" and \"gradient\")"},
         "cholesky_build_s": t_build,
         "dims": {k: d[k] for k in ("nvecs", "norb_pairs", "nshell_pairs", "nsubsets", "max_subset_dim",
                                    "npivot_shell_pairs", "nmacro")},
@@ -279,6 +332,8 @@ def main():
                      "flops": eri_flops, "kernel_s": eri_s},
     }
     stat_rows = sorted(ctx.eri_class_stats(), key=lambda r: -r[3]) if args.eri_stats else []
+    if not args.no_precision:
+        out["precision"] = precision_form(ctx, basis, Cmo, hip)
     ctx.close()
     if not args.no_batched:
         out["batched"] = batched_form(args.batch_workers)

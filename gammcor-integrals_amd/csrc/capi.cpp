@@ -358,6 +358,14 @@ int gi_chol_vdiag(gi_context *cp, double *dst) {
   })
 }
 
+int gi_chol_residual(gi_context *cp, double *max_residual, double *min_residual) {
+  GI_TRY({
+    gi::Context &C = ctx_of(cp);
+    if (!max_residual || !min_residual) throw Error{8, "null output pointer"};
+    gi::chol_residual(C, max_residual, min_residual);
+  })
+}
+
 int gi_chol_timings(gi_context *cp, double t[16]) {
   GI_TRY({
     gi::Context &C = ctx_of(cp);

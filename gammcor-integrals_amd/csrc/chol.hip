@@ -127,13 +127,18 @@ __device__ void block_argmax(double &k, double &v, long long &i) {
 
 // gather the batch block Mb(e, c) = M(bd(e), c), the batch diagonals, and the
 // initial first maximum
+// With Mb (precision form) the batch diagonals are taken from the updated
+// block itself, so that a pivot's residual vanishes within the recurrence, and
+// rows chosen in earlier macro iterations (excl) start at zero: the split
+// trailing update leaves them O(2^-24 |V|) instead of exactly zero.
 __global__ void __launch_bounds__(1024) k_micro_init(const double *__restrict__ D, const long long *__restrict__ bd,
                                                       long long nb, double inv_delta, double *__restrict__ Db,
-                                                      MicroState *st, int nsteps) {
+                                                      MicroState *st, int nsteps, const double *__restrict__ Mb,
+                                                      const unsigned char *__restrict__ excl) {
   double bk = inv_delta > 0.0 ? qabs(-1.0, inv_delta) : -1.0, bv = -1.0;
   long long bi = -1;
   for (long long e = threadIdx.x; e < nb; e += blockDim.x) {
-    double v = D[bd[e]];
+    double v = Mb ? (excl[e] ? 0.0 : Mb[e + nb * e]) : D[bd[e]];
     Db[e] = v;
     double k = inv_delta > 0.0 ? qabs(v, inv_delta) : v;
     argmax_merge(bk, bv, bi, k, v, e);
@@ -251,6 +256,13 @@ __global__ void k_micro_update(const MicroState *st, double *__restrict__ A, con
   double s = A[i + nb * c];
   for (int q = p0; q < j; ++q) s -= Lb[i + nb * q] * Lb[c + nb * q];
   A[i + nb * c] = s;
+}
+
+// D(bd(piv(q))) = 0 for the pivots of this macro iteration (precision form)
+__global__ void k_zero_pivot_d(double *__restrict__ D, const long long *__restrict__ bd, const int *__restrict__ piv,
+                               long long J) {
+  const long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (q < J) D[bd[piv[q]]] = 0.0;
 }
 
 // Lt(:, j0+q) = M(:, piv(q))  (right-hand sides of the triangular solve)
@@ -446,6 +458,64 @@ static void factor_and_invert(Context &C, DBuf<double> &V, long long n) {
   GI_HIP(hipStreamSynchronize(C.stream));
 }
 
+// r(pq) = Vdiag(full(pq)) - sum_k R(k, col)^2 for every Rkpq column; one
+// wavefront per column (256 lanes = 4 columns per workgroup), padding columns
+// (full index -1) give 0
+__global__ void __launch_bounds__(256) k_residual_diag(const double *__restrict__ R, long long nvec,
+                                                       const long long *__restrict__ colfull, long long ncol,
+                                                       const double *__restrict__ vdiag, double *__restrict__ out) {
+  const long long col = blockIdx.x * 4LL + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (col >= ncol) return;
+  double s = 0.0;
+  for (long long k = lane; k < nvec; k += 64) {
+    const double v = R[k + nvec * col];
+    s += v * v;
+  }
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+  if (lane == 0) out[col] = colfull[col] >= 0 ? vdiag[colfull[col]] - s : 0.0;
+}
+
+void chol_residual(Context &C, double *rmax, double *rmin) {
+  const CholState &S = C.chol;
+  if (!S.valid || !S.rkpq.p) throw Error{50, "Cholesky vectors are not available (run gi_chol_build first)"};
+  const long long msd = S.max_subset_dim, ncol = msd * S.nsubsets;
+  // Vdiag is stored shell pair by shell pair in chol2_Vdiag order (ShB <= ShA,
+  // ShA fastest), each pair's orbital pairs in the order of its Rkpq columns
+  const int nsh = C.nshells;
+  std::vector<long long> vdloc((size_t)nsh * nsh, 0);
+  for (long long sb = 0, loc = 0; sb < nsh; ++sb)
+    for (long long sa = sb; sa < nsh; ++sa) {
+      vdloc[sa * nsh + sb] = loc;
+      loc += pair_dim_of(C, (int)sa, (int)sb);
+    }
+  std::vector<long long> colfull(ncol, -1);   // column -> Vdiag index
+  for (long long s = 0; s < S.nshell_pairs; ++s) {
+    const long long A = S.shell_pairs[2 * s] - 1, B = S.shell_pairs[2 * s + 1] - 1;
+    const long long pos = S.shell_pair_loc[3 * s + 1] - 1, X = S.shell_pair_loc[3 * s + 2];
+    for (long long q = 0; q < S.shell_pair_dim[s]; ++q) colfull[(X - 1) * msd + pos + q] = vdloc[A * nsh + B] + q;
+  }
+  DBuf<long long> dcol;
+  DBuf<double> dv, dout;
+  dcol.upload(colfull, C.stream);
+  dv.upload(S.vdiag, C.stream);
+  dout.alloc((size_t)ncol);
+  hipLaunchKernelGGL(k_residual_diag, dim3((unsigned)((ncol + 3) / 4)), dim3(256), 0, C.stream, S.rkpq.p, S.nvecs,
+                     dcol.p, ncol, dv.p, dout.p);
+  GI_HIP(hipGetLastError());
+  std::vector<double> r(ncol);
+  GI_HIP(hipMemcpyAsync(r.data(), dout.p, sizeof(double) * ncol, hipMemcpyDeviceToHost, C.stream));
+  GI_HIP(hipStreamSynchronize(C.stream));
+  double mx = -INFINITY, mn = INFINITY;
+  for (long long c = 0; c < ncol; ++c)
+    if (colfull[c] >= 0) {
+      mx = std::max(mx, r[c]);
+      mn = std::min(mn, r[c]);
+    }
+  *rmax = mx;
+  *rmin = mn;
+}
+
 // ------------------------------------------------------------------ driver
 void chol_build(Context &C, double tau, double kappa, long long max_block_dim, long long block_dim,
                 long long flags) {
@@ -634,6 +704,7 @@ void chol_build(Context &C, double tau, double kappa, long long max_block_dim, l
     for (double v : Dh) dmax = std::max(dmax, v);
     bool do_macro = max_nchol > 0;
     DBuf<long long> dsrc, dloc, dbd;
+    DBuf<unsigned char> dexcl;
     DBuf<int> ddim;
     std::vector<double> mv;
     std::vector<long long> im;
@@ -766,8 +837,12 @@ void chol_build(Context &C, double tau, double kappa, long long max_block_dim, l
         hipLaunchKernelGGL(k_gather_rows_of_cols, dim3((unsigned)((nops_batch + 255) / 256), gy), dim3(256), 0,
                            C.stream, dLB.p, nops_batch, dL.p, ncand, dbd.p, nops_batch, nchol);
         const double alpha = -1.0, beta = 1.0;
-        ROCBLAS_CHECK(rocblas_dgemm(C.blas, rocblas_operation_none, rocblas_operation_transpose, ncand, nops_batch,
-                                    nchol, &alpha, dL.p, ncand, dLB.p, nops_batch, &beta, dM.p, ncand));
+        if (flags & 16)
+          split_bf16_gemm(C, rocblas_operation_none, rocblas_operation_transpose, ncand, nops_batch, nchol, alpha,
+                          dL.p, ncand, dLB.p, nops_batch, beta, dM.p, ncand);
+        else
+          ROCBLAS_CHECK(rocblas_dgemm(C.blas, rocblas_operation_none, rocblas_operation_transpose, ncand,
+                                      nops_batch, nchol, &alpha, dL.p, ncand, dLB.p, nops_batch, &beta, dM.p, ncand));
         GI_HIP(hipStreamSynchronize(C.stream));
         t_rtr += now() - tr;
         rtr_flops += 2.0 * (double)ncand * (double)nops_batch * (double)nchol;
@@ -792,8 +867,14 @@ void chol_build(Context &C, double tau, double kappa, long long max_block_dim, l
         hipLaunchKernelGGL(k_gather_block, dim3((unsigned)((nb + 255) / 256), (unsigned)nb), dim3(256), 0, C.stream,
                            dMb.p, dM.p, ncand, dbd.p, nb);
         const double qd = raw_ties ? 0.0 : inv_delta;
+        const bool split = (flags & 16) != 0;
+        if (split) {
+          std::vector<unsigned char> ex(nb);
+          for (long long e = 0; e < nb; ++e) ex[e] = pivots[bfull[e]] ? 1 : 0;
+          dexcl.upload(ex, C.stream);
+        }
         hipLaunchKernelGGL(k_micro_init, dim3(1), dim3(1024), 0, C.stream, dD.p, dbd.p, nb, qd, dDb.p, dstate.p,
-                           nsteps);
+                           nsteps, split ? dMb.p : nullptr, split ? dexcl.p : nullptr);
         // panels until the loop stops; the state is read back every 4 panels
         const int npanel_max = (nsteps + MICRO_PANEL - 1) / MICRO_PANEL;
         for (int pnl = 0; pnl < npanel_max;) {
@@ -833,6 +914,9 @@ void chol_build(Context &C, double tau, double kappa, long long max_block_dim, l
                              j0, J, dLb.p, nb, dbd.p, dD.p, ncand);
           hipLaunchKernelGGL(k_update_d, dim3((unsigned)((ncand + 255) / 256)), dim3(256), 0, C.stream, dL.p,
                              ncand, j0, J, dD.p, ncand);
+          if (split)
+            hipLaunchKernelGGL(k_zero_pivot_d, dim3((unsigned)((J + 255) / 256)), dim3(256), 0, C.stream, dD.p,
+                               dbd.p, dpiv.p, J);
         }
         nchol += J;
       }

@@ -153,6 +153,15 @@ void eri_run(Context &C, const EriRequest &R);
 // one-time device tables
 void upload_constants(Context &C);
 
+// C = beta C + alpha op(A) op(B) from split-BF16 products with FP32
+// accumulation (split.hip)
+void split_bf16_gemm(Context &C, rocblas_operation ta, rocblas_operation tb, long long m, long long n, long long k,
+                     double alpha, const double *A, long long lda, const double *B, long long ldb, double beta,
+                     double *Cd, long long ldc);
+
+// max and min over BASE orbital pairs of (pq|pq) - sum_k Rkpq(k, pq)^2 (chol.hip)
+void chol_residual(Context &C, double *rmax, double *rmin);
+
 // one md_kernel<E, F> launch (eri.hip)
 void launch_md_class(int E, int F, const MdArgs &a, int grid, hipStream_t s);
 

@@ -158,8 +158,12 @@ void mo_transform(Context &C, const double *dCA, long long lda, long long a0, lo
                      C.stream, S.rkpq.p, nvec, S.mo_adj_off.p, S.mo_adj_col.p, S.mo_adj_p.p, dCT.p, ldct, NA, dW.p);
   GI_HIP(hipGetLastError());
   const double one = 1.0, zero = 0.0;
-  RB_CHECK(rocblas_dgemm(C.blas, rocblas_operation_none, rocblas_operation_none, nvec * NA, NB, nao, &one, dW.p,
-                         nvec * NA, dCB + (b0 - 1) * ldb, ldb, &zero, dRkab, nvec * NA));
+  if (flags & 2)
+    split_bf16_gemm(C, rocblas_operation_none, rocblas_operation_none, nvec * NA, NB, nao, 1.0, dW.p, nvec * NA,
+                    dCB + (b0 - 1) * ldb, ldb, 0.0, dRkab, nvec * NA);
+  else
+    RB_CHECK(rocblas_dgemm(C.blas, rocblas_operation_none, rocblas_operation_none, nvec * NA, NB, nao, &one, dW.p,
+                           nvec * NA, dCB + (b0 - 1) * ldb, ldb, &zero, dRkab, nvec * NA));
   if (!(flags & 1)) {
     std::vector<long long> skip;
     for (long long a = 0; a < NA; ++a)

@@ -83,6 +83,11 @@ module gigpu_c
       real(c_double), value :: tau, kappa
       integer(c_int64_t), intent(out) :: status(*)
     end function
+    integer(c_int) function gi_chol_residual(ctx, max_residual, min_residual) bind(C, name="gi_chol_residual")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: max_residual, min_residual
+    end function
     integer(c_int) function gi_coulomb_energy_grad(ctx, density, ldd, kappa, energy, grad) &
         bind(C, name="gi_coulomb_energy_grad")
       import :: c_int, c_ptr, c_int64_t, c_double

@@ -17,13 +17,16 @@ EXPORTS = ["gi_create", "gi_destroy", "gi_last_error", "gi_set_basis", "gi_eri_q
            "gi_chol_build", "gi_chol_accuracy_tau", "gi_chol_dims", "gi_chol_meta", "gi_chol_rkpq",
            "gi_chol_rkpq_device", "gi_chol_inv_l", "gi_chol_vdiag", "gi_chol_timings",
            "gi_mo_transform", "gi_mo_transform_device", "gi_eri_class_stats", "gi_eri_pair_block",
-           "gi_set_option", "gi_chol_build_batch", "gi_coulomb_energy_grad"]
+           "gi_set_option", "gi_chol_build_batch", "gi_coulomb_energy_grad",
+           "gi_chol_residual"]
 
 CHOL_DECODE_FIXED = 1
 CHOL_SKIP_VECTORS = 2
 CHOL_TIME_ERI = 4
 CHOL_RAW_TIES = 8
+CHOL_SPLIT_BF16 = 16
 MO_ALL_COLUMNS = 1
+MO_SPLIT_BF16 = 2
 
 i64 = C.c_int64
 dptr = C.POINTER(C.c_double)
@@ -63,6 +66,7 @@ def load():
     L.gi_set_option.argtypes = [C.c_void_p, C.c_char_p, i64]
     L.gi_chol_build_batch.argtypes = [i64, C.POINTER(C.c_void_p), C.c_double, C.c_double, i64, i64, i64, i64,
                                       iptr]
+    L.gi_chol_residual.argtypes = [C.c_void_p, dptr, dptr]
     L.gi_coulomb_energy_grad.argtypes = [C.c_void_p, dptr, i64, C.c_double, dptr, C.c_void_p]
     L.gi_mo_transform.argtypes = [C.c_void_p, dptr, i64, i64, i64, dptr, i64, i64, i64, i64, i64, dptr]
     L.gi_mo_transform_device.argtypes = [C.c_void_p, C.c_void_p, i64, i64, i64, C.c_void_p, i64, i64, i64, i64,
@@ -195,6 +199,12 @@ class Context:
         out = np.zeros(n * n)
         _check(self.L.gi_chol_inv_l(self.h, _p(out)))
         return out.reshape(n, n).T
+
+    def chol_residual(self):
+        """(max, min) over BASE orbital pairs of (pq|pq) - sum_k Rkpq(k, pq)^2."""
+        mx, mn = np.zeros(1), np.zeros(1)
+        _check(self.L.gi_chol_residual(self.h, _p(mx), _p(mn)))
+        return float(mx[0]), float(mn[0])
 
     def coulomb_energy(self, density, kappa=0.0, grad=True):
         """E_J = 1/2 sum D_pq (pq|rs) D_rs of a symmetric AO density (nao x nao),

@@ -39,11 +39,17 @@ typedef struct gi_context gi_context;
                                    raw doubles, as the reference does, instead of
                                    keys quantized to 2^-40 (relative for the BASE
                                    sort, of max(Vdiag) for residual diagonals) */
+#define GI_CHOL_SPLIT_BF16 16   /* precision form: the Step-1 trailing update
+                                   (chol2_SubtractR) as split-BF16 products with
+                                   FP32 accumulation instead of FP64 dgemm */
 
 /* gi_mo_transform flags */
 #define GI_MO_ALL_COLUMNS 1     /* transform every a column even when a0 > 1
                                    (the reference's pass bounds,
                                    Cholesky_Gammcor.f90:196-197, skip some) */
+#define GI_MO_SPLIT_BF16 2      /* precision form: the second transform step
+                                   (Rab = W CB) as split-BF16 products with FP32
+                                   accumulation instead of FP64 dgemm */
 
 /* Create a context on HIP device `device` (no reference counterpart: the
  * reference initialises module state in chol_gammcor_Rkpq,
@@ -149,6 +155,11 @@ int gi_chol_rkpq_device(gi_context *ctx, double **ptr);
 /* Inv_L(NVecs, NVecs) and Vdiag(NAO(NAO+1)/2) to host memory. */
 int gi_chol_inv_l(gi_context *ctx, double *dst);
 int gi_chol_vdiag(gi_context *ctx, double *dst);
+/* Reconstruction error of the last build on its diagonal: over the BASE
+ * orbital pairs pq, r(pq) = (pq|pq) - sum_k Rkpq(k, pq)^2, the quantity the
+ * decomposition bounds by tau (TwoStepCholesky_Step1.f90:1004-1064 stops when
+ * the residual diagonal is below tau).  Returns its maximum and minimum. */
+int gi_chol_residual(gi_context *ctx, double *max_residual, double *min_residual);
 /* Phase timings in seconds: [0] step 1, [1] step 2, [2] full vectors,
  * [3] total build, [4] ERI kernel seconds (GI_CHOL_TIME_ERI), [5] ERI FP64
  * operations, [6] Vdiag, [7] step-1 ERIs, [8] M - R^T R, [9] micro
