@@ -317,32 +317,26 @@ __global__ void k_blockmax(const double *__restrict__ D, long long n, double *__
   if (threadIdx.x == 0) part[blockIdx.x] = sv[0];
 }
 
-// grid (ceil(n / 256), min(n, 65535)), 256 lanes; columns grid-stride in y
-__global__ void k_zero_upper(double *A, long long n) {
-  for (long long c = blockIdx.y; c < n; c += gridDim.y)
-    for (long long r = threadIdx.x + blockIdx.x * (long long)blockDim.x; r < c && r < n;
-         r += (long long)gridDim.x * blockDim.x)
-      A[r + n * c] = 0.0;
-}
 
 // ---- V = L L^T and L^-1 of the pivot block (real_Cholesky, real_linalg.f90:39-57,
 // and real_LowerTriangularInverse, :60-72, called at TwoStepCholesky.f90:344-350).
 // LAPACK dpotrf/dtrtri semantics: lower triangle, info = the order of the
-// first leading minor that is not positive definite.  Right-looking blocked
-// Cholesky: a POTF_NB-wide diagonal block is factored in LDS by one workgroup
-// (k_potf2_block), the panel below it is solved with dtrsm and the trailing
-// matrix updated with one dgemm.  The inverse solves L X = I column block by
-// column block, each block only on the rows at and below its diagonal.
+// first leading minor that is not positive definite.  Recursive: a diagonal
+// block of at most POTF_NB rows is factored and inverted in LDS by one
+// workgroup (k_chol_inv_leaf); above that, with the block split in halves,
+//   L21 = A21 X11^T,   A22 -= L21 L21^T,   (L22, X22) of A22,   X21 = -X22 (L21 X11),
+// X = L^-1, four dgemms per split.  Only X is kept (it is Inv_L).
 constexpr int POTF_NB = 64;
-constexpr int TRTRI_W = 512;
 
 __global__ void k_set_int(int *p, int v) { *p = v; }
 
-// unblocked Cholesky of A(k0:k0+b, k0:k0+b), b <= POTF_NB; does nothing once
-// an earlier block has failed
-__global__ void __launch_bounds__(256) k_potf2_block(double *__restrict__ A, long long lda, long long k0, int b,
-                                                     int *__restrict__ info) {
+// Cholesky factor L and its inverse X of the diagonal block at k0 (b <= POTF_NB
+// rows, lower triangle of V read); writes L (zero above the diagonal) over the
+// block of V and X into the block of X.  Does nothing once an earlier block failed.
+__global__ void __launch_bounds__(256) k_chol_inv_leaf(double *__restrict__ V, double *__restrict__ X, long long n,
+                                                       long long k0, int b, int *__restrict__ info) {
   __shared__ double s[POTF_NB][POTF_NB + 1];
+  __shared__ double x[POTF_NB][POTF_NB + 1];
   __shared__ int s_info;
   const int tid = threadIdx.x;
   if (tid == 0) s_info = *info;
@@ -350,7 +344,8 @@ __global__ void __launch_bounds__(256) k_potf2_block(double *__restrict__ A, lon
   if (s_info != 0) return;
   for (int e = tid; e < b * b; e += 256) {
     const int r = e % b, c = e / b;
-    s[r][c] = r >= c ? A[(k0 + r) + lda * (k0 + c)] : 0.0;
+    s[r][c] = r >= c ? V[(k0 + r) + n * (k0 + c)] : 0.0;
+    x[r][c] = 0.0;
   }
   __syncthreads();
   int jend = b;
@@ -371,18 +366,26 @@ __global__ void __launch_bounds__(256) k_potf2_block(double *__restrict__ A, lon
     }
     __syncthreads();
   }
+  if (jend < b) {
+    if (tid == 0) *info = (int)(k0 + jend + 1);
+    return;
+  }
+  // X = L^-1 by forward substitution, one column per lane
+  if (tid < b) {
+    const int c = tid;
+    x[c][c] = 1.0 / s[c][c];
+    for (int r = c + 1; r < b; ++r) {
+      double acc = 0.0;
+      for (int k = c; k < r; ++k) acc += s[r][k] * x[k][c];
+      x[r][c] = -acc / s[r][r];
+    }
+  }
+  __syncthreads();
   for (int e = tid; e < b * b; e += 256) {
     const int r = e % b, c = e / b;
-    if (r >= c) A[(k0 + r) + lda * (k0 + c)] = s[r][c];
+    V[(k0 + r) + n * (k0 + c)] = r >= c ? s[r][c] : 0.0;
+    X[(k0 + r) + n * (k0 + c)] = r >= c ? x[r][c] : 0.0;
   }
-  if (tid == 0 && jend < b) *info = (int)(k0 + jend + 1);
-}
-
-// same grid as k_zero_upper
-__global__ void k_identity(double *X, long long n) {
-  const long long r = threadIdx.x + blockIdx.x * (long long)blockDim.x;
-  if (r >= n) return;
-  for (long long c = blockIdx.y; c < n; c += gridDim.y) X[r + n * c] = r == c ? 1.0 : 0.0;
 }
 
 // ------------------------------------------------------------------ helpers
@@ -416,46 +419,54 @@ struct EriPhaseTimer {
   }
 };
 
-// V (n x n, lower triangle significant) -> L^-1 in place; throws the
-// reference's errors when V is not positive definite
+// factor and invert the diagonal block [k0, k0 + m) of V (n x n); X (zero on
+// entry) receives L^-1 in the same block; W: scratch of (m/2 + POTF_NB)^2
+static void chol_inv_rec(Context &C, double *V, double *X, long long n, long long k0, long long m, double *W,
+                         int *dinfo) {
+  if (m <= POTF_NB) {
+    hipLaunchKernelGGL(k_chol_inv_leaf, dim3(1), dim3(256), 0, C.stream, V, X, n, k0, (int)m, dinfo);
+    return;
+  }
+  const long long m1 = (m / 2 + POTF_NB - 1) / POTF_NB * POTF_NB, m2 = m - m1;
+  chol_inv_rec(C, V, X, n, k0, m1, W, dinfo);
+  double *A21 = V + (k0 + m1) + n * k0, *A22 = V + (k0 + m1) + n * (k0 + m1);
+  double *X11 = X + k0 + n * k0, *X21 = X + (k0 + m1) + n * k0, *X22 = X + (k0 + m1) + n * (k0 + m1);
+  const double one = 1.0, mone = -1.0, zero = 0.0;
+  // L21 = A21 X11^T, stored over A21
+  ROCBLAS_CHECK(rocblas_dgemm(C.blas, rocblas_operation_none, rocblas_operation_transpose, m2, m1, m1, &one, A21, n,
+                              X11, n, &zero, W, m2));
+  GI_HIP(hipMemcpy2DAsync(A21, sizeof(double) * n, W, sizeof(double) * m2, sizeof(double) * m2, m1,
+                          hipMemcpyDeviceToDevice, C.stream));
+  // A22 -= L21 L21^T (the whole square; only its lower triangle is read later)
+  ROCBLAS_CHECK(rocblas_dgemm(C.blas, rocblas_operation_none, rocblas_operation_transpose, m2, m2, m1, &mone, A21, n,
+                              A21, n, &one, A22, n));
+  chol_inv_rec(C, V, X, n, k0 + m1, m2, W, dinfo);
+  // X21 = -X22 (L21 X11)
+  ROCBLAS_CHECK(rocblas_dgemm(C.blas, rocblas_operation_none, rocblas_operation_none, m2, m1, m1, &one, A21, n, X11,
+                              n, &zero, W, m2));
+  ROCBLAS_CHECK(rocblas_dgemm(C.blas, rocblas_operation_none, rocblas_operation_none, m2, m1, m2, &mone, X22, n, W,
+                              m2, &zero, X21, n));
+}
+
+// V (n x n, lower triangle significant) -> L^-1; throws the reference's error
+// when V is not positive definite
 static void factor_and_invert(Context &C, DBuf<double> &V, long long n) {
   DBuf<int> dinfo;
   dinfo.alloc(1);
   hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, C.stream, dinfo.p, 0);
-  const double one = 1.0, mone = -1.0;
-  for (long long k0 = 0; k0 < n; k0 += POTF_NB) {
-    const int b = (int)std::min<long long>(POTF_NB, n - k0);
-    hipLaunchKernelGGL(k_potf2_block, dim3(1), dim3(256), 0, C.stream, V.p, n, k0, b, dinfo.p);
-    const long long m = n - k0 - b;
-    if (m > 0) {
-      double *Lkk = V.p + k0 + n * k0, *A21 = V.p + (k0 + b) + n * k0, *A22 = V.p + (k0 + b) + n * (k0 + b);
-      ROCBLAS_CHECK(rocblas_dtrsm(C.blas, rocblas_side_right, rocblas_fill_lower, rocblas_operation_transpose,
-                                  rocblas_diagonal_non_unit, m, b, &one, Lkk, n, A21, n));
-      ROCBLAS_CHECK(rocblas_dgemm(C.blas, rocblas_operation_none, rocblas_operation_transpose, m, m, b, &mone, A21,
-                                  n, A21, n, &one, A22, n));
-    }
-  }
+  DBuf<double> X, W;
+  X.alloc((size_t)n * n);
+  GI_HIP(hipMemsetAsync(X.p, 0, sizeof(double) * (size_t)n * n, C.stream));
+  const long long h = n / 2 + POTF_NB;
+  W.alloc((size_t)h * h);
+  chol_inv_rec(C, V.p, X.p, n, 0, n, W.p, dinfo.p);
   int info = 0;
   GI_HIP(hipMemcpyAsync(&info, dinfo.p, sizeof(info), hipMemcpyDeviceToHost, C.stream));
   GI_HIP(hipStreamSynchronize(C.stream));
   if (info != 0) throw Error{40, "Cholesky factorization failed with exit code info=" + std::to_string(info)};
-  const dim3 gsq((unsigned)((n + 255) / 256), (unsigned)std::min<long long>(n, 65535));
-  hipLaunchKernelGGL(k_zero_upper, gsq, dim3(256), 0, C.stream, V.p, n);
-  // X = L^-1: column block [c0, c0 + w) is zero above row c0 and solves
-  // L(c0:, c0:) X(c0:, c0:c0+w) = I(c0:, c0:c0+w)
-  DBuf<double> X;
-  X.alloc((size_t)n * n);
-  hipLaunchKernelGGL(k_identity, gsq, dim3(256), 0, C.stream, X.p, n);
-  for (long long c0 = 0; c0 < n; c0 += TRTRI_W) {
-    const long long w = std::min<long long>(TRTRI_W, n - c0);
-    ROCBLAS_CHECK(rocblas_dtrsm(C.blas, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none,
-                                rocblas_diagonal_non_unit, n - c0, w, &one, V.p + c0 + n * c0, n,
-                                X.p + c0 + n * c0, n));
-  }
   std::swap(V.p, X.p);
   std::swap(V.n, X.n);
   std::swap(V.st, X.st);
-  GI_HIP(hipStreamSynchronize(C.stream));
 }
 
 // r(pq) = Vdiag(full(pq)) - sum_k R(k, col)^2 for every Rkpq column; one
