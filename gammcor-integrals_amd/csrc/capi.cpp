@@ -88,6 +88,7 @@ int gi_set_basis(gi_context *cp, int64_t natoms, const double *coords, int64_t n
     C.shell_np.assign(nshells, 0);
     C.shell_loc.assign(nshells, 0);
     C.shell_atom.assign(nshells, 0);
+    C.atom_xyz.assign(coords, coords + 3 * natoms);
     std::vector<double> hexp, hcn, hnorm;
     int loc = 0;
     for (int64_t s = 0; s < nshells; ++s) {
@@ -260,6 +261,27 @@ int gi_coulomb_energy_grad(gi_context *cp, const double *density, int64_t ldd, d
   GI_TRY({
     gi::Context &C = ctx_of(cp);
     gi::coulomb_energy_grad(C, density, ldd, kappa, energy, grad);
+  })
+}
+
+int gi_one_electron(gi_context *cp, const double *atom_charges, int64_t npc, const double *pc_coords,
+                    const double *pc_charges, double *S, double *T, double *V, int64_t ldh) {
+  GI_TRY({
+    gi::Context &C = ctx_of(cp);
+    gi::one_electron(C, atom_charges, npc, pc_coords, pc_charges, S, T, V, ldh);
+  })
+}
+
+int gi_h0(gi_context *cp, const double *atom_charges, int64_t npc, const double *pc_coords, const double *pc_charges,
+          double *H0, int64_t ldh) {
+  GI_TRY({
+    gi::Context &C = ctx_of(cp);
+    if (!H0) throw Error{8, "null H0"};
+    const long long n = C.nao;
+    std::vector<double> Tm((size_t)n * n);
+    gi::one_electron(C, atom_charges, npc, pc_coords, pc_charges, nullptr, Tm.data(), H0, ldh);
+    for (long long j = 0; j < n; ++j)
+      for (long long i = 0; i < n; ++i) H0[i + ldh * j] += Tm[i + n * j];
   })
 }
 

@@ -11,6 +11,7 @@
 #include <unordered_map>
 #include "gi_context.h"
 #include "grad_kernels.h"
+#include "onee_kernels.h"
 
 namespace gi {
 

@@ -95,6 +95,7 @@ struct Context {
   std::vector<DevShell> hshells;
   std::vector<int> shell_l, shell_nang, shell_np, shell_loc;   // shell_loc: 0-based first AO
   std::vector<int> shell_atom;                                  // 0-based atom of each shell
+  std::vector<double> atom_xyz;                                 // (3, natoms) bohr
   DBuf<DevShell> dshells;
   DBuf<double> dexps, dcntr, dnorms, dboys;
   bool basis_set = false;
@@ -175,6 +176,13 @@ struct GradJob {
 };
 void launch_build_grad_et(const Context &C, const int *ao_loc, const double *D, long long ldd,
                           const GradJob *jobs, long long njobs, PrimRec *prims, double *et);
+
+// one-electron integrals (onee_kernels.h with eri.hip, onee.hip)
+constexpr int ONEE_LMAX_HOST = 3;
+void launch_one_electron(const Context &C, int L, const int *ao_loc, const int *pairs, long long npairs,
+                         const double *chg, int ncharge, double *S, double *T, double *V, long long ldh);
+void one_electron(Context &C, const double *atom_charges, long long npc, const double *pc_xyz, const double *pc_q,
+                  double *S, double *T, double *V, long long ldh);
 
 // Coulomb energy of a density and its nuclear gradient (grad.hip)
 void coulomb_energy_grad(Context &C, const double *D, long long ldd, double kappa, double *energy,

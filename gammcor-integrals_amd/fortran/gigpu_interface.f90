@@ -83,6 +83,20 @@ module gigpu_c
       real(c_double), value :: tau, kappa
       integer(c_int64_t), intent(out) :: status(*)
     end function
+    integer(c_int) function gi_one_electron(ctx, atom_charges, npc, pc_coords, pc_charges, S, T, V, ldh) &
+        bind(C, name="gi_one_electron")
+      import :: c_int, c_ptr, c_int64_t
+      type(c_ptr), value :: ctx, atom_charges, pc_coords, pc_charges, S, T, V
+      integer(c_int64_t), value :: npc, ldh
+    end function
+    integer(c_int) function gi_h0(ctx, atom_charges, npc, pc_coords, pc_charges, H0, ldh) &
+        bind(C, name="gi_h0")
+      import :: c_int, c_ptr, c_int64_t, c_double
+      type(c_ptr), value :: ctx, pc_coords, pc_charges
+      real(c_double), intent(in) :: atom_charges(*)
+      integer(c_int64_t), value :: npc, ldh
+      real(c_double), intent(out) :: H0(ldh, *)
+    end function
     integer(c_int) function gi_chol_residual(ctx, max_residual, min_residual) bind(C, name="gi_chol_residual")
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: ctx

@@ -18,7 +18,7 @@ EXPORTS = ["gi_create", "gi_destroy", "gi_last_error", "gi_set_basis", "gi_eri_q
            "gi_chol_rkpq_device", "gi_chol_inv_l", "gi_chol_vdiag", "gi_chol_timings",
            "gi_mo_transform", "gi_mo_transform_device", "gi_eri_class_stats", "gi_eri_pair_block",
            "gi_set_option", "gi_chol_build_batch", "gi_coulomb_energy_grad",
-           "gi_chol_residual"]
+           "gi_chol_residual", "gi_one_electron", "gi_h0"]
 
 CHOL_DECODE_FIXED = 1
 CHOL_SKIP_VECTORS = 2
@@ -67,6 +67,9 @@ def load():
     L.gi_chol_build_batch.argtypes = [i64, C.POINTER(C.c_void_p), C.c_double, C.c_double, i64, i64, i64, i64,
                                       iptr]
     L.gi_chol_residual.argtypes = [C.c_void_p, dptr, dptr]
+    L.gi_one_electron.argtypes = [C.c_void_p, C.c_void_p, i64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                  C.c_void_p, i64]
+    L.gi_h0.argtypes = [C.c_void_p, C.c_void_p, i64, C.c_void_p, C.c_void_p, dptr, i64]
     L.gi_coulomb_energy_grad.argtypes = [C.c_void_p, dptr, i64, C.c_double, dptr, C.c_void_p]
     L.gi_mo_transform.argtypes = [C.c_void_p, dptr, i64, i64, i64, dptr, i64, i64, i64, i64, i64, dptr]
     L.gi_mo_transform_device.argtypes = [C.c_void_p, C.c_void_p, i64, i64, i64, C.c_void_p, i64, i64, i64, i64,
@@ -199,6 +202,37 @@ class Context:
         out = np.zeros(n * n)
         _check(self.L.gi_chol_inv_l(self.h, _p(out)))
         return out.reshape(n, n).T
+
+    @staticmethod
+    def _charges(natoms, charges, point_charges):
+        q = None if charges is None else np.ascontiguousarray(np.asarray(charges, dtype=np.float64))
+        if q is not None and q.shape != (natoms,):
+            raise ValueError("one charge per atom")
+        pc = np.zeros((0, 4)) if point_charges is None else \
+            np.ascontiguousarray(np.asarray(point_charges, dtype=np.float64).reshape(-1, 4))
+        xyz = np.ascontiguousarray(pc[:, :3])
+        qq = np.ascontiguousarray(pc[:, 3])
+        return q, len(pc), xyz, qq
+
+    def one_electron(self, charges, point_charges=None):
+        """(S, T, V) in the AO basis: gi_one_electron.  charges: one per atom;
+        point_charges: rows (x, y, z, q), bohr."""
+        n = self.basis.nao
+        q, npc, xyz, qq = self._charges(self.basis.natoms, charges, point_charges)
+        S, T, V = (np.zeros((n, n), order="F") for _ in range(3))
+        _check(self.L.gi_one_electron(self.h, None if q is None else q.ctypes.data, npc,
+                                      xyz.ctypes.data if npc else None, qq.ctypes.data if npc else None,
+                                      S.ctypes.data, T.ctypes.data, V.ctypes.data, n))
+        return S, T, V
+
+    def h0(self, charges, point_charges=None):
+        """H0 = T + V_ne in the AO basis: gi_h0 (ints1e_gammcor_H0)."""
+        n = self.basis.nao
+        q, npc, xyz, qq = self._charges(self.basis.natoms, charges, point_charges)
+        H = np.zeros((n, n), order="F")
+        _check(self.L.gi_h0(self.h, None if q is None else q.ctypes.data, npc, xyz.ctypes.data if npc else None,
+                            qq.ctypes.data if npc else None, _p(H), n))
+        return H
 
     def chol_residual(self):
         """(max, min) over BASE orbital pairs of (pq|pq) - sum_k Rkpq(k, pq)^2."""

@@ -127,6 +127,20 @@ int gi_chol_build_batch(int64_t n, gi_context *const *ctxs, double tau, double k
  * for every shell pair. */
 int gi_coulomb_energy_grad(gi_context *ctx, const double *density, int64_t ldd, double kappa, double *energy,
                            double *grad);
+/* One-electron integrals in the AO basis of the context (column-major,
+ * leading dimension ldh >= nao, both triangles filled): the overlap S
+ * (ints1e_S, OneElectronInts.f90:915-932), the kinetic energy T (ints1e_T,
+ * :935-953) and the nuclear attraction V (ints1e_Vne, :956-975), any of them
+ * NULL to skip.  atom_charges[natoms] are the nuclear charges the reference
+ * takes from TSystem (ZNumbers or ZNumbersECP, 0 for dummy atoms; NULL for
+ * none), and pc_coords (3 x npc, bohr) / pc_charges (npc) the external point
+ * charges (TSystem%PointChargeCoords / PointCharges).  Shells up to l = 3. */
+int gi_one_electron(gi_context *ctx, const double *atom_charges, int64_t npc, const double *pc_coords,
+                    const double *pc_charges, double *S, double *T, double *V, int64_t ldh);
+/* The bare-nucleus Hamiltonian H0 = T + V_ne in the AO basis
+ * (ints1e_gammcor_H0, OneElectronInts_Gammcor.f90:13-49); arguments as above. */
+int gi_h0(gi_context *ctx, const double *atom_charges, int64_t npc, const double *pc_coords,
+          const double *pc_charges, double *H0, int64_t ldh);
 /* tau of a CHOL_ACCURACY_* level; returns 0 and sets *tau, or fails with
  * "Invalid accuracy setting" (Cholesky_Gammcor.f90:407-409). */
 int gi_chol_accuracy_tau(int64_t accuracy, double *tau);

@@ -160,6 +160,12 @@ static void rtuv(double *R, double *tmp, int L, const double *F, double alpha,
 #undef IX
 }
 
+/* exported for onee.c: R_tuv as rtuv(), R and tmp (L+1)^3 each */
+void or_rtuv(double *R, double *tmp, int64_t L, const double *F, double alpha, double X, double Y, double Z)
+{
+    rtuv(R, tmp, (int)L, F, alpha, X, Y, Z);
+}
+
 /* horizontal recurrence (Head-Gordon & Pople eq. 18):
  * (c, d+1_i) = (c+1_i, d) + (C_i - D_i)(c, d), direction = first nonzero of d
  * in x,y,z order (Auto2e.py:137-177).

@@ -119,6 +119,12 @@ int64_t or_mo_transform(const or_basis *b, const or_chol_result *res, int64_t na
                         const double *CB, int64_t ldb, int64_t b0, int64_t b1,
                         int64_t max_buffer_mb, double *Rkab);
 
+/* ---- One-electron integrals (onee.c) ---- */
+/* S, T, V_ne in the AO basis (nao x nao, leading dim ldh, any may be NULL);
+ * V from the atom charges (natoms, dummy atoms 0) and npc point charges. */
+void or_one_electron(const or_basis *b, const double *atom_charges, int64_t npc, const double *pc_xyz,
+                     const double *pc_q, double *S, double *T, double *V, int64_t ldh);
+
 /* ---- Coulomb energy of an AO density (coulomb.c) ---- */
 /* E_J = 1/2 sum D_pq (pq|rs) D_rs; D column-major (nao x nao, leading dim ldd). */
 double or_coulomb_energy(const or_basis *b, const double *D, int64_t ldd, double kappa);

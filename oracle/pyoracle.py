@@ -73,6 +73,8 @@ def lib():
         L.or_nang.restype = i64
         L.or_eri_quartet.argtypes = [C.POINTER(OrBasis), i64, i64, i64, i64, C.c_double, dptr]
         L.or_eri_quartet_single_boys.argtypes = [C.POINTER(OrBasis), i64, i64, i64, i64, C.c_double, dptr]
+        L.or_one_electron.argtypes = [C.POINTER(OrBasis), dptr, i64, dptr, dptr, C.c_void_p, C.c_void_p,
+                                      C.c_void_p, i64]
         L.or_coulomb_energy.argtypes = [C.POINTER(OrBasis), dptr, i64, C.c_double]
         L.or_coulomb_energy.restype = C.c_double
         L.or_dsort0.argtypes = [dptr, iptr, i64, i64]
@@ -119,6 +121,19 @@ class Oracle:
         f = self.L.or_eri_quartet_single_boys if single_boys else self.L.or_eri_quartet
         f(C.byref(self.ob), sa, sb, sc, sd, kappa, _p(V))
         return V.reshape(n[3], n[2], n[1], n[0]).transpose(3, 2, 1, 0)
+
+    def one_electron(self, charges, point_charges=None):
+        """(S, T, V) in the AO basis; charges per atom, point_charges an
+        iterable of (x, y, z, q) in bohr."""
+        n = self.basis.nao
+        S, T, V = (np.zeros((n, n), order="F") for _ in range(3))
+        q = np.ascontiguousarray(np.asarray(charges, dtype=np.float64))
+        pc = np.zeros((0, 4)) if point_charges is None else np.asarray(point_charges, dtype=np.float64).reshape(-1, 4)
+        xyz = np.ascontiguousarray(pc[:, :3]) if len(pc) else np.zeros(3)
+        qq = np.ascontiguousarray(pc[:, 3]) if len(pc) else np.zeros(1)
+        self.L.or_one_electron(C.byref(self.ob), _p(q), len(pc), _p(xyz), _p(qq), S.ctypes.data, T.ctypes.data,
+                               V.ctypes.data, n)
+        return S, T, V
 
     def coulomb_energy(self, density, kappa=0.0):
         """E_J = 1/2 sum D_pq (pq|rs) D_rs of a symmetric AO density."""
