@@ -59,3 +59,21 @@ def test_dummy_atoms_carry_no_charge(ctx):
     assert np.abs(V0).max() == 0.0
     _, T1, V1 = ctx.one_electron(None)
     assert np.abs(V1).max() == 0.0 and np.abs(T1).max() > 0.0
+
+
+def test_gammcor_module_mirror(ctx):
+    from gigpu import gammcor, oneelectron
+    basis = systems.water_system(2, 5.0, "small", seed=34)
+    system = oneelectron.system_of(basis)
+    assert list(system.atom_charges()) == [8.0, 1.0, 1.0] * 2
+    H = oneelectron.ints1e_gammcor_H0(None, basis, system, ctx)
+    _, To, Vo = pyoracle.Oracle(basis).one_electron(system.atom_charges())
+    assert np.abs(H - (To + Vo)).max() <= 1e-12 * np.abs(To + Vo).max()
+    rng = np.random.default_rng(5)
+    C, _ = np.linalg.qr(rng.normal(size=(basis.nao, basis.nao)))
+    Hmo = oneelectron.ints1e_gammcor_H0_mo(None, C[:, :12], basis, system, gammcor.ORBITAL_ORDERING_DALTON, ctx)
+    assert np.abs(Hmo - C[:, :12].T @ H @ C[:, :12]).max() <= 1e-12 * np.abs(Hmo).max()
+    with pytest.raises(gammcor.GammcorError, match="Invalid dimension of matrix C"):
+        oneelectron.ints1e_gammcor_H0_mo(None, C[:-1], basis, system, gammcor.ORBITAL_ORDERING_DALTON, ctx)
+    with pytest.raises(gammcor.GammcorError, match="not supported"):
+        oneelectron.ints1e_gammcor_H0_extao(None, basis, system, gammcor.ORBITAL_ORDERING_ORCA, ctx)
