@@ -150,7 +150,7 @@ static void launch_lane(const MdArgs &a, int grid, hipStream_t s) {
 }
 
 static LaunchFn md_table[MD_EMAX + 1][MD_EMAX + 1];
-static LaunchFn lane_table[LANE_EMAX + 1][LANE_EMAX + 1];
+static LaunchFn lane_table[MD_EMAX + 1][MD_EMAX + 1];   // null where a class has no lane kernel
 
 static void init_md_table() {
   static bool done = false;
@@ -337,12 +337,20 @@ void eri_run(Context &C, const EriRequest &R) {
     if (nrows == 0 || ncols == 0) continue;
     const long long nq = (R.mode == LAUNCH_PRODUCT) ? nrows * ncols : nrows;
     // lane-per-quartet kernel for small uncontracted classes
-    bool lane_path = C.eri_lane && (L.E <= LANE_EMAX && L.F <= LANE_EMAX);
+    // the lane kernels want the thin side (lower Hermite order) as their bra:
+    // a product with E > F runs transposed on the (F, E) kernel
+    bool trans = false;
+    bool lane_path = false;
+    if (C.eri_lane && std::max(L.E, L.F) <= C.eri_lane_emax && L.E <= MD_EMAX && L.F <= MD_EMAX) {
+      const bool lt = R.mode == LAUNCH_PRODUCT && C.eri_trans && L.E > L.F && lane_table[L.F][L.E] != nullptr;
+      lane_path = lt || lane_table[L.E][L.F] != nullptr;
+      trans = lt;
+    }
     for (size_t k = 0; lane_path && k < L.rows.size(); ++k) lane_path = nprim_of(rd[L.rows[k]]) == 1;
     for (size_t k = 0; lane_path && k < L.cols.size(); ++k) lane_path = nprim_of(cd[L.cols[k]]) == 1;
+    if (!lane_path) trans = false;
     // MFMA kernel orientation: its bra side (stage-2 rows, 16-row tiles) is
     // the side with more rows per pair; the thin side is stacked in columns
-    bool trans = false;
     if (!lane_path && R.mode == LAUNCH_PRODUCT && C.eri_trans) {
       double sr = 0, sc = 0;
       for (int r : L.rows) sr += nrow_of(rd[r]);

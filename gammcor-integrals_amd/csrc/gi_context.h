@@ -131,6 +131,7 @@ struct Context {
   // launch-layer tuning (gi_set_option)
   long long eri_min_tiles = 8192, eri_max_kt = 32;
   bool eri_lane = true, eri_trans = true;
+  long long eri_lane_emax = LANE_EMAX;   // lane kernels only for classes with E, F <= this
 
   Context(int dev);
   ~Context();

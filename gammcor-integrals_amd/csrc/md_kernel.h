@@ -490,13 +490,13 @@ __global__ void __launch_bounds__(64) md_kernel(MdArgs A) {
 }
 
 // ------------------------------------------------------------- lane kernel
-// Small classes (la+lb <= 2, lc+ld <= 2, uncontracted pairs): one lane per
+// Small and thin-bra classes (the list in md_classes.inc; uncontracted pairs): one lane per
 // quartet, everything in registers.  A wave takes 64 consecutive bra pairs of
 // the row list against one ket pair (PRODUCT; the ket side is then uniform
 // across the wave and its loads scalar), or 64 consecutive zipped pairs.
 //   y(k)    = sum_h ET_bra(h, r) R_{h+k}
 //   V(r, c) = sum_k (-1)^{|k|} y(k) ET_ket(k, c)
-constexpr int LANE_EMAX = 2;
+constexpr int LANE_EMAX = 4;   // largest E or F with a lane kernel (md_classes.inc lists them)
 
 __host__ __device__ constexpr int hsum(int h, int k) {
   return hidx(px(herm_of(h)) + px(herm_of(k)), py(herm_of(h)) + py(herm_of(k)), pz(herm_of(h)) + pz(herm_of(k)));
@@ -588,6 +588,8 @@ __global__ void __launch_bounds__(256) md_lane_kernel(MdArgs A) {
     const int ldb = RD.n0 * RD.n1, ldk = CD.n0 * CD.n1;
     const double *etb = A.et + PB.et, *etk = A.et + PK.et;
     double *dst = A.out + RD.off + CD.off;
+    // transposed launch: the kernel's bra rows are the destination's columns
+    const long long rs = A.trans ? A.ldo : 1, cs = A.trans ? 1 : A.ldo;
     double R[nherm(L)];
     rtable_reg<L>(R, PB, PK, A.kappa, A.boys);
 #pragma unroll 1
@@ -620,7 +622,7 @@ __global__ void __launch_bounds__(256) md_lane_kernel(MdArgs A) {
               double v = 0.0;
 #pragma unroll
               for (int k = 0; k < NHF; ++k) v += y[k] * etk[(long long)k * ldk + fc];
-              dst[r + A.ldo * c] = v;
+              dst[rs * r + cs * c] = v;
             }
           }
         }
