@@ -1103,7 +1103,7 @@ void chol_build(Context &C, double tau, double kappa, long long max_block_dim, l
       // block row r only needs the first r1 columns of Inv_L, so the product
       // is one GEMM per block row ((nblk + 1) / (2 nblk) of the dense FLOPs)
       const double one = 1.0, zero = 0.0;
-      const long long nblk = std::max<long long>(1, std::min<long long>(8, nvec / 512));
+      const long long nblk = std::max<long long>(1, std::min<long long>(C.vec_blocks, nvec / 512));
       for (long long rb = 0; rb < nblk; ++rb) {
         const long long r0 = nvec * rb / nblk, r1 = nvec * (rb + 1) / nblk;
         if (r1 <= r0) continue;

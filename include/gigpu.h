@@ -96,8 +96,9 @@ int gi_eri_pair_block(gi_context *ctx, int64_t nrow_pairs, const int64_t *row_pa
  *                    kernels, 0: every class on the MFMA kernels (1)
  *   "eri_trans"      1: MFMA launches put the side with more rows per pair
  *                    on the 16-row tiles, 0: never transpose (1)
- *   "eri_lane_emax"  lane kernels only for classes with E, F <= this (4;
- *                    kernels exist for E, F <= 4 with E + F <= 6)
+ *   "eri_lane_emax"  lane kernels only for classes with E, F <= this (4)
+ *   "vec_blocks"     block rows of the triangular Inv_L Vabrs^T product of
+ *                    the full-dimension vectors (8)
  * Unknown names fail.  Results do not depend on these settings beyond
  * floating-point rounding. */
 int gi_set_option(gi_context *ctx, const char *name, int64_t value);
