@@ -78,10 +78,12 @@ def profiled_traffic():
     return None
 
 
-def batched_form(nworkers: int):
+def batched_form(nworkers: int, autotune_batch: bool = True):
     """The batched form on 64 independent molecules ((H2O)n, n uniform in
     1..12, O 6s4p2d / H 3s1p): the same decompositions one at a time (plain)
-    and concurrently (gi_chol_build_batch), after one warm-up pass."""
+    and concurrently (gi_chol_build_batch), after one warm-up pass; then the
+    autotuner's settings for this workload (tile sizes, lane kernels, vector
+    blocks, workers) and the batched time with them."""
     from gigpu import lib, systems
     mols = systems.batch_systems(64, seed=3)
     ctxs = [lib.Context(0) for _ in mols]
@@ -99,6 +101,17 @@ def batched_form(nworkers: int):
         lib.chol_build_batch(ctxs, tau=1e-6, nworkers=nworkers)
         t_batch = time.perf_counter() - t0
         nv_batch = [c.chol_dims()["nvecs"] for c in ctxs]
+        tuned = None
+        if autotune_batch:
+            from gigpu import autotune
+            best, _, trials = autotune.tune(ctxs, nworkers=[nworkers, 8, 16], reps=1)
+            ts = []
+            for _ in range(2):
+                t0 = time.perf_counter()
+                lib.chol_build_batch(ctxs, tau=1e-6, nworkers=best["nworkers"])
+                ts.append(time.perf_counter() - t0)
+            tuned = {"settings": best, "batched_s": float(min(ts)), "trials": len(trials),
+                     "same_nvecs": [c.chol_dims()["nvecs"] for c in ctxs] == nv_plain}
     finally:
         for c in ctxs:
             c.close()
@@ -106,7 +119,8 @@ def batched_form(nworkers: int):
             "nao_total": int(sum(b.nao for b in mols)), "nvecs_total": int(sum(nv_batch)),
             "same_nvecs": nv_plain == nv_batch, "quartets": int(quartets), "workers": nworkers,
             "plain_s": t_plain, "batched_s": t_batch, "speedup": t_plain / t_batch,
-            "plain_quartets_per_s": quartets / t_plain, "batched_quartets_per_s": quartets / t_batch}
+            "plain_quartets_per_s": quartets / t_plain, "batched_quartets_per_s": quartets / t_batch,
+            "autotuned": tuned}
 
 
 def precision_form(ctx, basis, Cmo, hip, reps: int = 3):
@@ -221,6 +235,7 @@ def main():
     ap.add_argument("--no-mo", action="store_true")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-form comparison")
     ap.add_argument("--batch-workers", type=int, default=12)
+    ap.add_argument("--no-autotune", action="store_true", help="skip autotuning the batched form")
     ap.add_argument("--no-gradient", action="store_true", help="skip the differentiation-form timing")
     ap.add_argument("--no-precision", action="store_true", help="skip the split-BF16 precision-form comparison")
     ap.add_argument("--eri-stats", action="store_true",
@@ -341,7 +356,7 @@ def main():
         out["precision"] = precision_form(ctx, basis, Cmo, hip)
     ctx.close()
     if not args.no_batched:
-        out["batched"] = batched_form(args.batch_workers)
+        out["batched"] = batched_form(args.batch_workers, not args.no_autotune)
     if not args.no_gradient:
         out["gradient"] = gradient_form()
     if not args.no_cpu_baseline:

@@ -82,3 +82,25 @@ def test_full_batch_repeated_under_load():
     finally:
         for c in ctxs:
             c.close()
+
+
+def test_autotuner_picks_from_candidates_and_keeps_results():
+    from gigpu import autotune
+    mols = systems.batch_systems(nmol=6, seed=8, nmin=1, nmax=3)
+    ctxs = [lib.Context(0) for _ in mols]
+    try:
+        for c, b in zip(ctxs, mols):
+            c.set_basis(b)
+        lib.chol_build_batch(ctxs, tau=1e-6, nworkers=2)
+        ref = [c.chol_dims()["nvecs"] for c in ctxs]
+        cand = {"eri_max_kt": [32, 8], "eri_lane_emax": [4, 2]}
+        best, t, trials = autotune.tune(ctxs, candidates=cand, nworkers=[2, 4], reps=1)
+        assert best["eri_max_kt"] in (32, 8) and best["eri_lane_emax"] in (4, 2)
+        assert best["nworkers"] in (2, 4)
+        assert t == min(tt for _, tt in trials)
+        assert len(trials) == 4
+        lib.chol_build_batch(ctxs, tau=1e-6, nworkers=best["nworkers"])
+        assert [c.chol_dims()["nvecs"] for c in ctxs] == ref
+    finally:
+        for c in ctxs:
+            c.close()
