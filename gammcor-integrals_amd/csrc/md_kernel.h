@@ -177,7 +177,8 @@ __device__ __forceinline__ int pair_full(const QRec &d, const int *sel, int r) {
 // slot NHL.  Ends with a barrier.
 template <int L>
 __device__ __forceinline__ void rtable(double *Ra, double *Rb, const PrimRec &pb, const PrimRec &pk,
-                                       double kappa, const double *__restrict__ boys, int lane) {
+                                       double kappa, const double *__restrict__ boys, int lane,
+                                       const int *__restrict__ rrec) {
   const double PQx = pb.px - pk.px, PQy = pb.py - pk.py, PQz = pb.pz - pk.pz;
   const double pq = pb.p * pk.p;
   const double den = pb.p + pk.p + kappa * pq;
@@ -211,7 +212,7 @@ __device__ __forceinline__ void rtable(double *Ra, double *Rb, const PrimRec &pb
         if (idx == 0) {
           r = Fm[n];
         } else {
-          const int rec = c_rrec[idx];
+          const int rec = rrec[idx];
           const int i1 = rec & 0x7ff, i2 = (rec >> 11) & 0x7ff, dir = (rec >> 22) & 3, coef = rec >> 24;
           const double pqd = dir == 0 ? PQx : (dir == 1 ? PQy : PQz);
           r = (double)coef * prv[i2] + pqd * prv[i1];
@@ -250,6 +251,7 @@ __global__ void __launch_bounds__(64) md_kernel(MdArgs A) {
   __shared__ double s_X[HT * 16][MD_RING];
   __shared__ long long s_qbase[MD_RING];
   __shared__ double s_E[NHF][MD_NCV];
+  __shared__ int s_rrec[NHL];   // R recurrence codes, read by every R table of the launch
 
   const int lane = threadIdx.x;
   const int i16 = lane & 15, g = lane >> 4;
@@ -269,6 +271,7 @@ __global__ void __launch_bounds__(64) md_kernel(MdArgs A) {
     s_idx[e] = v;
   }
   for (int e = lane; e < HT * 16 * MD_RING; e += 64) (&s_X[0][0])[e] = 0.0;
+  for (int e = lane; e < NHL; e += 64) s_rrec[e] = c_rrec[e];
   if (lane == 0) {
     s_R[0][NHL] = 0.0;
     s_R[1][NHL] = 0.0;
@@ -372,7 +375,7 @@ __global__ void __launch_bounds__(64) md_kernel(MdArgs A) {
         const int nc = CD.nrow;
         const int ldk = CD.n0 * CD.n1;
         const bool single = (CD.nprim == 1);
-        if (single) rtable<E + F>(s_R[0], s_R[1], PB, first_prim(CD), A.kappa, A.boys, lane);
+        if (single) rtable<E + F>(s_R[0], s_R[1], PB, first_prim(CD), A.kappa, A.boys, lane, s_rrec);
 
         for (int c0 = 0; c0 < nc; c0 += 16) {
           const int cc = c0 + i16;
@@ -387,7 +390,7 @@ __global__ void __launch_bounds__(64) md_kernel(MdArgs A) {
               for (int c = 0; c < MD_NCV; ++c) xv[ps][c] = 0.0;
             for (int kp = 0; kp < CD.nprim; ++kp) {
               const PrimRec PK = kp == 0 ? first_prim(CD) : A.prim[CD.prim0 + kp];
-              if (!single) rtable<E + F>(s_R[0], s_R[1], PB, PK, A.kappa, A.boys, lane);
+              if (!single) rtable<E + F>(s_R[0], s_R[1], PB, PK, A.kappa, A.boys, lane, s_rrec);
               const double *etk = A.et + PK.et;
               for (int e = lane; e < NHF * MD_NCV; e += 64) {
                 const int k = e / MD_NCV, c = e - k * MD_NCV;
@@ -434,7 +437,7 @@ __global__ void __launch_bounds__(64) md_kernel(MdArgs A) {
           for (int ht = 0; ht < HT; ++ht) xacc[ht] = d4{0.0, 0.0, 0.0, 0.0};
           for (int kp = 0; kp < CD.nprim; ++kp) {
             const PrimRec PK = kp == 0 ? first_prim(CD) : A.prim[CD.prim0 + kp];
-            if (!single) rtable<E + F>(s_R[0], s_R[1], PB, PK, A.kappa, A.boys, lane);
+            if (!single) rtable<E + F>(s_R[0], s_R[1], PB, PK, A.kappa, A.boys, lane, s_rrec);
             const double *etk = A.et + PK.et;
             const double *Rf = s_R[0];
 #pragma unroll U1
