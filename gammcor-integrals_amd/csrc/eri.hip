@@ -320,8 +320,7 @@ void eri_run(Context &C, const EriRequest &R) {
     hipLaunchKernelGGL(k_pack_recs, dim3((unsigned)((idx.size() + 255) / 256)), dim3(256), 0, C.stream, C.md_rdesc.p,
                        C.md_cdesc.p, C.md_idx.p, C.md_side.p, (long long)idx.size(), C.md_hdr.p, C.md_prim.p,
                        C.md_rec.p);
-    GI_HIP(hipGetLastError());
-    ++nlaunch;
+    GI_HIP(hipGetLastError());   // record packing: not counted among the ERI kernel launches
   }
 
   static const bool stats = getenv("GIGPU_ERI_STATS") != nullptr;
