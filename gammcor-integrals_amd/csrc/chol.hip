@@ -337,14 +337,15 @@ __global__ void __launch_bounds__(256) k_chol_inv_leaf(double *__restrict__ V, d
                                                        long long k0, int b, int *__restrict__ info) {
   __shared__ double s[POTF_NB][POTF_NB + 1];
   __shared__ double x[POTF_NB][POTF_NB + 1];
+  __shared__ double t[POTF_NB * POTF_NB / 2];   // B A^-1 of the merge step
   __shared__ int s_info;
   const int tid = threadIdx.x;
   if (tid == 0) s_info = *info;
   __syncthreads();
   if (s_info != 0) return;
-  for (int e = tid; e < b * b; e += 256) {
-    const int r = e % b, c = e / b;
-    s[r][c] = r >= c ? V[(k0 + r) + n * (k0 + c)] : 0.0;
+  for (int e = tid; e < POTF_NB * POTF_NB; e += 256) {
+    const int r = e % POTF_NB, c = e / POTF_NB;
+    s[r][c] = (r < b && c < b && r >= c) ? V[(k0 + r) + n * (k0 + c)] : (r == c ? 1.0 : 0.0);
     x[r][c] = 0.0;
   }
   __syncthreads();
@@ -370,17 +371,41 @@ __global__ void __launch_bounds__(256) k_chol_inv_leaf(double *__restrict__ V, d
     if (tid == 0) *info = (int)(k0 + jend + 1);
     return;
   }
-  // X = L^-1 by forward substitution, one column per lane
-  if (tid < b) {
-    const int c = tid;
-    x[c][c] = 1.0 / s[c][c];
-    for (int r = c + 1; r < b; ++r) {
+  // X = L^-1 (L padded to POTF_NB rows with an identity): the eight 8 x 8
+  // diagonal blocks by forward substitution, one column per lane, then pairs
+  // of blocks merged, [A 0; B C]^-1 = [A^-1 0; -C^-1 (B A^-1) C^-1], for
+  // h = 8, 16, 32, each product spread over all lanes (short dependency chains)
+  if (tid < POTF_NB) {
+    const int r0 = (tid / 8) * 8, c = tid % 8;
+    x[r0 + c][r0 + c] = 1.0 / s[r0 + c][r0 + c];
+    for (int r = c + 1; r < 8; ++r) {
       double acc = 0.0;
-      for (int k = c; k < r; ++k) acc += s[r][k] * x[k][c];
-      x[r][c] = -acc / s[r][r];
+      for (int k = c; k < r; ++k) acc += s[r0 + r][r0 + k] * x[r0 + k][r0 + c];
+      x[r0 + r][r0 + c] = -acc / s[r0 + r][r0 + r];
     }
   }
   __syncthreads();
+  for (int h = 8; h < POTF_NB; h *= 2) {
+    const int npair = POTF_NB / (2 * h);
+    // T = B A^-1 for every pair (A^-1 lower triangular)
+    for (int e = tid; e < npair * h * h; e += 256) {
+      const int pr = e / (h * h), ij = e % (h * h), i = ij % h, j = ij / h;
+      const int a0 = 2 * h * pr, c0 = a0 + h;
+      double acc = 0.0;
+      for (int k = j; k < h; ++k) acc += s[c0 + i][a0 + k] * x[a0 + k][a0 + j];
+      t[e] = acc;
+    }
+    __syncthreads();
+    // X21 = -C^-1 T (C^-1 lower triangular)
+    for (int e = tid; e < npair * h * h; e += 256) {
+      const int pr = e / (h * h), ij = e % (h * h), i = ij % h, j = ij / h;
+      const int a0 = 2 * h * pr, c0 = a0 + h;
+      double acc = 0.0;
+      for (int k = 0; k <= i; ++k) acc += x[c0 + i][c0 + k] * t[pr * h * h + k + h * j];
+      x[c0 + i][a0 + j] = -acc;
+    }
+    __syncthreads();
+  }
   for (int e = tid; e < b * b; e += 256) {
     const int r = e % b, c = e / b;
     V[(k0 + r) + n * (k0 + c)] = r >= c ? s[r][c] : 0.0;
