@@ -64,14 +64,20 @@ __global__ void k_segmax(const double *__restrict__ D, const long long *__restri
 // Near-tie canonicalization of comparison keys (same rule as the oracle):
 // values equal in exact arithmetic differ in their last bits with a sign set
 // by each implementation's rounding; comparing quantized keys makes the
-// pivot order a function of the mathematics.  BASE sort keys: mantissa
-// rounded to 40 bits.  Residual diagonals: absolute quantum
-// delta = 2^(floor(log2 MaxDrs) - 40), a power of two so x/delta is exact.
+// pivot order a function of the mathematics.  The quanta sit well above the
+// disagreement of two correct ERI evaluations (Boys-function and summation
+// order differences reach 1e-11 relative on small diagonals, 1e-13 absolute
+// on residuals) and well below tau.  BASE sort keys: mantissa rounded to
+// KEY_REL_BITS bits.  Residual diagonals: absolute quantum
+// delta = 2^(floor(log2 MaxDrs) - KEY_ABS_BITS), a power of two so x/delta is
+// exact.
+constexpr int KEY_REL_BITS = 26;
+constexpr int KEY_ABS_BITS = 32;
 static double qrel(double x) {
   if (x == 0.0 || !std::isfinite(x)) return x;
   int e;
   double m = std::frexp(x, &e);
-  return std::ldexp(std::rint(std::ldexp(m, 41)), e - 41);
+  return std::ldexp(std::rint(std::ldexp(m, KEY_REL_BITS + 1)), e - KEY_REL_BITS - 1);
 }
 __host__ __device__ static inline double qabs(double x, double inv_delta) { return rint(x * inv_delta); }
 
@@ -640,7 +646,7 @@ void chol_build(Context &C, double tau, double kappa, long long max_block_dim, l
   const bool raw_ties = (flags & 8) != 0;
   int maxdrs_exp = 0;
   std::frexp(maxdrs, &maxdrs_exp);
-  const double inv_delta = std::ldexp(1.0, 41 - maxdrs_exp);   // 1 / 2^(floor(log2 MaxDrs) - 40)
+  const double inv_delta = std::ldexp(1.0, KEY_ABS_BITS + 1 - maxdrs_exp);   // 1 / delta
   if (!raw_ties)
     for (long long k = 1; k <= max_sp; ++k) vd_max[k] = qrel(vd_max[k]);
   dsort0(vd_max.data() + 1, idxmap.data() + 1, max_sp, 2);

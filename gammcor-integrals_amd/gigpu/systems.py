@@ -4,7 +4,8 @@ The workloads are water clusters (H2O)n: O atoms uniform in a cube, rejection
 sampled to a minimum O-O distance, each H2O rigid (O-H 0.96 A, HOH 104.5 deg)
 in a uniformly random orientation.  Bases are uncontracted even-tempered
 shells, alpha_k = alpha0 * beta**k, alpha0 ~ U[0.1, 0.3] drawn per
-(element, l), contraction coefficient 1.0.
+(element, l), contraction coefficient 1.0; `contracted_shells` builds
+segmented contractions over the same kind of primitive set.
 
 Writers for the reference's input formats are provided so the same system can
 be fed to the reference driver: XYZ (sys_definitions.f90:588-723) and the
@@ -72,10 +73,35 @@ def even_tempered_shells(spec: dict, seed: int, beta: float = 2.5):
     return shells
 
 
+def contracted_shells(spec: dict, seed: int, beta: float = 2.5):
+    """Segmented contractions over an even-tempered primitive set: per
+    (element, l) with `count` shells, the first shell contracts the three
+    tightest primitives, the second the next two, the rest are single
+    primitives; coefficients ~ U[0.2, 0.8].  Exercises the primitive-pair
+    loops that uncontracted shells leave at one iteration."""
+    rng = np.random.default_rng(seed)
+    shells = {}
+    for el in sorted(spec):
+        lst = []
+        for l, count in enumerate(spec[el]):
+            a0 = rng.uniform(0.1, 0.3)
+            nprims = [3, 2] + [1] * max(0, count - 2)
+            nprims = nprims[:count]
+            alphas = [a0 * beta ** k for k in range(sum(nprims))][::-1]
+            pos = 0
+            for npr in nprims:
+                ex = alphas[pos:pos + npr]
+                pos += npr
+                lst.append((l, ex, list(rng.uniform(0.2, 0.8, npr))))
+        shells[el] = lst
+    return shells
+
+
 def water_system(n: int, box_angstrom: float, basis: str, seed: int,
-                 spher_ao: bool = True) -> AOBasis:
+                 spher_ao: bool = True, contracted: bool = False) -> AOBasis:
     symbols, xyz = water_cluster(n, box_angstrom, seed)
-    shells = even_tempered_shells(BASIS_SPECS[basis], seed + 1000003)
+    make = contracted_shells if contracted else even_tempered_shells
+    shells = make(BASIS_SPECS[basis], seed + 1000003)
     return build_basis(symbols, xyz / BOHR_ANGSTROM, shells, spher_ao=spher_ao)
 
 

@@ -37,8 +37,8 @@ typedef struct gi_context gi_context;
 #define GI_CHOL_TIME_ERI 4      /* time ERI launches with HIP events */
 #define GI_CHOL_RAW_TIES 8      /* compare the sort keys and the batch argmax on
                                    raw doubles, as the reference does, instead of
-                                   keys quantized to 2^-40 (relative for the BASE
-                                   sort, of max(Vdiag) for residual diagonals) */
+                                   keys quantized to 2^-26 relative (BASE sort) and
+                                   2^-32 of max(Vdiag) (residual diagonals) */
 #define GI_CHOL_SPLIT_BF16 16   /* precision form: the Step-1 trailing update
                                    (chol2_SubtractR) as split-BF16 products with
                                    FP32 accumulation instead of FP64 dgemm */

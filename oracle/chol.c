@@ -51,17 +51,24 @@ static double wall(void) { return omp_get_wtime(); }
  * To make the pivot sequence a function of the mathematics (and identical
  * between this oracle and the GPU library), keys are quantized before they
  * are compared:
- *   - diagonal maxima for the BASE sort: mantissa rounded to 40 bits;
+ *   - diagonal maxima for the BASE sort: mantissa rounded to KEY_REL_BITS
+ *     (26) bits;
  *   - residual diagonals (candidate/batch sorts, batch argmax): rounded to
- *     an absolute quantum delta = 2^(floor(log2 MaxDrs) - 40).
+ *     an absolute quantum delta = 2^(floor(log2 MaxDrs) - KEY_ABS_BITS),
+ *     KEY_ABS_BITS = 32.
+ * The quanta sit above the disagreement of two correct ERI evaluations
+ * (different Boys-function evaluation and summation order reach 1e-11
+ * relative on small diagonals) and far below tau.
  * Quantized ties fall back to the sort's tie order and the first-in-batch
  * rule.  OR_CHOL_RAW_TIES restores raw comparisons. */
+enum { KEY_REL_BITS = 26, KEY_ABS_BITS = 32 };
+
 static double qrel(double x)
 {
     if (x == 0.0 || !isfinite(x)) return x;
     int e;
     double m = frexp(x, &e);
-    return ldexp(rint(ldexp(m, 41)), e - 41);
+    return ldexp(rint(ldexp(m, KEY_REL_BITS + 1)), e - KEY_REL_BITS - 1);
 }
 
 static double qabs(double x, double inv_delta) { return rint(x * inv_delta); }
@@ -197,7 +204,7 @@ void or_chol_build(const or_basis *b, double tau, double kappa, int64_t max_bloc
     const int raw_ties = (flags & OR_CHOL_RAW_TIES) != 0;
     int maxdrs_exp;
     frexp(maxdrs, &maxdrs_exp);
-    const double inv_delta = ldexp(1.0, 41 - maxdrs_exp);   /* 1 / 2^(floor(log2 MaxDrs) - 40) */
+    const double inv_delta = ldexp(1.0, KEY_ABS_BITS + 1 - maxdrs_exp);   /* 1 / delta */
     if (!raw_ties)
         for (int64_t k = 1; k <= max_sp; ++k) vd_max[k] = qrel(vd_max[k]);
     or_dsort0(vd_max + 1, idxmap + 1, max_sp, 2);

@@ -48,6 +48,29 @@ def test_against_obara_saika(water):
         assert err <= 2e-11 * max(1.0, np.abs(R).max()), (ls, q, err)
 
 
+@pytest.mark.parametrize("spher", [True, False], ids=["spherical", "cartesian"])
+def test_contracted_shells_against_obara_saika(oracle_mod, spher):
+    """Segmented contractions (3- and 2-primitive shells mixed with single
+    primitives) in spherical and Cartesian AOs against the independent
+    evaluator: a contraction coefficient applied twice or not at all, a
+    primitive loop bound off by one, or a Cartesian norm factor on the wrong
+    component fails it."""
+    b = systems.water_system(1, 6.0, "large", seed=31, contracted=True, spher_ao=spher)
+    o = oracle_mod.Oracle(b)
+    multi = {l: [i for i in range(b.nshells) if b.shell_l(i) == l and b.nprimitives[b.shell_params_idx[i] - 1] > 1]
+             for l in range(4)}
+    single = {l: [i for i in range(b.nshells) if b.shell_l(i) == l] for l in range(4)}
+    rng = np.random.default_rng(6)
+    combos = [(0, 0, 0, 0), (1, 0, 1, 1), (2, 1, 0, 2), (3, 0, 1, 0), (2, 2, 3, 1), (1, 3, 2, 0)]
+    for k, ls in enumerate(combos):
+        q = [int(rng.choice(multi[l] if (j + k) % 2 == 0 and multi[l] else single[l])) for j, l in enumerate(ls)]
+        V = o.eri(*q)
+        R = ind.shell_quartet(b, q)
+        assert V.shape == R.shape
+        err = np.abs(V - R).max()
+        assert err <= 2e-11 * max(1.0, np.abs(R).max()), (ls, q, err)
+
+
 def test_kappa_attenuated_ssss(oracle_mod):
     # erf(omega r12)/r12: alpha -> PQ/(P+Q+kappa PQ), closed form for s functions
     b = build_basis(["H", "H"], np.array([[0.0, 0.0, 0.0], [0.0, 0.3, 1.4]]),
