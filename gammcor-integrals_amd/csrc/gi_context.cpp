@@ -51,6 +51,11 @@ Context::~Context() {
   chol.mo_adj_off.release();
   chol.mo_adj_col.release();
   chol.mo_adj_p.release();
+  for (int i = 0; i < 2; ++i) {
+    spl_slices[i].release();
+    spl_max[i].release();
+    spl_exp[i].release();
+  }
   // every buffer is released (stream-ordered frees on `stream`) before the
   // stream goes away
   if (stream) (void)hipStreamSynchronize(stream);

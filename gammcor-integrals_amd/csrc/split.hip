@@ -1,65 +1,260 @@
 // Split-BF16 products on MFMA: the precision form of the library's FP64
 // GEMMs (the Cholesky trailing update and the AO->MO transform).
 //
-// Every FP64 operand element is split into three BF16 parts,
-//   x = x1 + x2 + x3 + O(2^-24 |x|),  x1 = bf16(x), x2 = bf16(x - x1), x3 = bf16(x - x1 - x2),
-// and a product is assembled from the six part products whose size is above
-// 2^-24 of the leading one,
-//   A B = A1 B1 + (A1 B2 + A2 B1 + A1 B3 + A2 B2 + A3 B1) + O(2^-24 |A||B|),
-// each a rocBLAS gemm_ex with BF16 inputs and FP32 accumulation.  The five
-// correction products go into their own FP32 accumulator, so that their
-// rounding is not absorbed by the leading term, and both FP32 partials are
-// added to the FP64 destination (compensated sum).  The error of the result
-// is that of FP32 accumulation over k terms of |A||B|, which the callers
-// report against their FP64 path.
-#include <hip/hip_bf16.h>
+// Error-free splitting (Ozaki, Ogita, Oishi, Rump, Numer. Algorithms 59
+// (2012) 95-118) onto BF16 MFMA with FP32 accumulation:
+//  * row r of op(A) is scaled by 2^-e_r and column c of op(B) by 2^-f_c, the
+//    powers of two that bring the largest element of the row (column) into
+//    [1/2, 1);
+//  * a scaled element x is cut into four slices of eight bits,
+//      x = sum_t q_t 2^(-8t) + d,  |d| <= 2^-33,
+//    q_1 in [-256, 256] and q_t in [-128, 128] integers, exact in BF16;
+//  * A B is the sum over slice pairs (t, u) with t + u <= 5 (ten products,
+//    the omitted pairs are below 2^-33) of 2^(-8(t+u)) q_t p_u; the pairs of
+//    one level t + u share an FP32 accumulator;
+//  * q_t p_u is an integer below 2^16 in magnitude, so every partial sum of a
+//    level over 128 values of k is an integer below 2^24: exact in FP32
+//    whatever the order of the MFMA's additions.  After each 128 k the four
+//    level sums are scaled and added into an FP64 accumulator.
+// The only error is the truncation of the slices,
+//   |err(r, c)| <~ 2^-32 sum_k (max|A(r,:)| |B(k,c)| + |A(r,k)| max|B(:,c)|),
+// with no growth from the accumulation over k (plain FP32 accumulation adds
+// ~2^-24 sqrt(k) sum_k |A(r,k)||B(k,c)|).
 #include "gi_context.h"
 
 namespace gi {
 
-#define SPLIT_RB(x)                                                                                \
-  do {                                                                                           \
-    rocblas_status s_ = (x);                                                                     \
-    if (s_ != rocblas_status_success)                                                            \
-      throw Error{2, std::string("rocBLAS error ") + rocblas_status_to_string(s_) + " at " + __FILE__ + ":" + \
-                         std::to_string(__LINE__)};                                              \
-  } while (0)
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-// parts of an m x n column-major block (leading dimension ld) into three
-// contiguous m x n BF16 arrays
-__global__ void k_split3(const double *__restrict__ x, long long ld, long long m, long long n,
-                         __hip_bfloat16 *__restrict__ p1, __hip_bfloat16 *__restrict__ p2,
-                         __hip_bfloat16 *__restrict__ p3) {
-  const long long total = m * n;
-  for (long long e = threadIdx.x + blockIdx.x * (long long)blockDim.x; e < total;
-       e += (long long)gridDim.x * blockDim.x) {
-    const long long i = e % m, j = e / m;
-    const double v = x[i + ld * j];
-    const __hip_bfloat16 a = __float2bfloat16((float)v);
-    const double r1 = v - (double)__bfloat162float(a);
-    const __hip_bfloat16 b = __float2bfloat16((float)r1);
-    const double r2 = r1 - (double)__bfloat162float(b);
-    p1[e] = a;
-    p2[e] = b;
-    p3[e] = __float2bfloat16((float)r2);
+constexpr int SPL_S = 4;      // slices per element
+constexpr int SPL_KC = 128;   // k per exact FP32 chunk; k is padded to a multiple
+constexpr int SPL_BM = 64;    // workgroup tile rows: 2 waves x 32
+constexpr int SPL_BN = 128;   // workgroup tile columns: 2 waves x 64
+constexpr int SPL_T = 64;     // slicing tile (rows x k)
+
+// Element (r, k) of a slicing operand is X[r rs + k ks]; one of rs, ks is 1.
+// The 64 x 64 tile at (r0, k0) is read along the contiguous index into LDS,
+// zero outside rows x kd.
+__device__ static inline void load_tile(double (*t)[SPL_T + 1], const double *__restrict__ X, long long rs,
+                                        long long ks, long long rows, long long kd, long long r0, long long k0,
+                                        bool absval) {
+  const bool rfast = rs == 1;
+  for (int e = threadIdx.x; e < SPL_T * SPL_T; e += blockDim.x) {
+    const int a = e % SPL_T, b = e / SPL_T;
+    const int ri = rfast ? a : b, ki = rfast ? b : a;
+    const long long r = r0 + ri, k = k0 + ki;
+    double v = 0.0;
+    if (r < rows && k < kd) v = X[r * rs + k * ks];
+    t[ri][ki] = absval ? fabs(v) : v;
   }
 }
 
-// C = beta C + alpha (P + Q), C FP64 (ldc), P and Q FP32 contiguous m x n
-__global__ void k_add_partials(double *__restrict__ C, long long ldc, long long m, long long n, double alpha,
-                               double beta, const float *__restrict__ P, const float *__restrict__ Q) {
-  const long long total = m * n;
-  for (long long e = threadIdx.x + blockIdx.x * (long long)blockDim.x; e < total;
-       e += (long long)gridDim.x * blockDim.x) {
-    const long long i = e % m, j = e / m;
-    const double s = (double)P[e] + (double)Q[e];
-    double *c = C + i + ldc * j;
-    *c = beta == 0.0 ? alpha * s : beta * *c + alpha * s;
+// rmax[r] = max_k |X(r, k)|, as the bits of a non-negative double (their
+// unsigned order is the order of the values); rmax is zero on entry.
+__global__ __launch_bounds__(256) void k_row_absmax(const double *__restrict__ X, long long rs, long long ks,
+                                                    long long rows, long long kd,
+                                                    unsigned long long *__restrict__ rmax) {
+  __shared__ double t[SPL_T][SPL_T + 1];
+  const long long r0 = (long long)blockIdx.x * SPL_T, k0 = (long long)blockIdx.y * SPL_T;
+  load_tile(t, X, rs, ks, rows, kd, r0, k0, true);
+  __syncthreads();
+  if (threadIdx.x < SPL_T) {
+    double m = 0.0;
+    for (int i = 0; i < SPL_T; ++i) m = fmax(m, t[threadIdx.x][i]);
+    const long long r = r0 + threadIdx.x;
+    if (r < rows && m > 0.0) atomicMax(rmax + r, (unsigned long long)__double_as_longlong(m));
   }
 }
 
-static unsigned grid_for(long long total) {
-  return (unsigned)std::min<long long>((total + 255) / 256, 65536);
+// Slices of the rows r < rows_pad, k < kpad (zero outside rows x kd) in the
+// MFMA fragment order: unit = 8 BF16 (16 bytes), unit index
+//   ((t nrb + rb) nks + ks) 64 + l,   rb = r / 16, ks = k / 32,
+//   l = r % 16 + 16 ((k % 32) / 8),
+// so one wave reads the 16 x 32 fragment (rb, ks) of slice t as one
+// contiguous kilobyte.  The rows' scale exponents go to e.
+__global__ __launch_bounds__(256) void k_slice_rows(const double *__restrict__ X, long long rs, long long ks,
+                                                    long long rows, long long kd,
+                                                    const unsigned long long *__restrict__ rmax, long long rows_pad,
+                                                    long long kpad, bf16x8 *__restrict__ P, int *__restrict__ e) {
+  __shared__ double t[SPL_T][SPL_T + 1];
+  __shared__ int se[SPL_T];
+  const long long r0 = (long long)blockIdx.x * SPL_T, k0 = (long long)blockIdx.y * SPL_T;
+  load_tile(t, X, rs, ks, rows, kd, r0, k0, false);
+  if (threadIdx.x < SPL_T) {
+    const long long r = r0 + threadIdx.x;
+    int ex = 0;
+    if (r < rows) {
+      const double m = __longlong_as_double((long long)rmax[r]);
+      if (m > 0.0) (void)frexp(m, &ex);
+    }
+    se[threadIdx.x] = ex;
+    if (blockIdx.y == 0) e[r] = ex;
+  }
+  __syncthreads();
+  const long long nrb = rows_pad / 16, nks = kpad / 32;
+  const size_t plane = (size_t)nrb * (size_t)nks * 64;
+  // 4 row blocks x 2 k steps x 64 lanes of 8 k each in the tile
+  for (int i = threadIdx.x; i < 512; i += blockDim.x) {
+    const int l = i & 63, kb = (i >> 6) & 1, rbl = i >> 7;
+    const int ri = 16 * rbl + (l & 15), kq = 32 * kb + 8 * (l >> 4);
+    const size_t u = ((size_t)(r0 / 16 + rbl) * (size_t)nks + (size_t)(k0 / 32 + kb)) * 64 + (size_t)l;
+    double a[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) a[j] = ldexp(t[ri][kq + j], 8 - se[ri]);   // |a| < 256
+#pragma unroll
+    for (int s = 0; s < SPL_S; ++s) {
+      bf16x8 v;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const double q = rint(a[j]);
+        v[j] = (__bf16)(float)q;
+        a[j] = (a[j] - q) * 256.0;                 // exact: |a - q| <= 1/2
+      }
+      P[s * plane + u] = v;
+    }
+  }
+}
+
+// C(r, c) = beta C + alpha 2^(ea[r] + eb[c]) sum_k x_r(k) y_c(k), x and y the
+// slice sums of the rows of A (mpad rows) and B (npad rows), both in the
+// fragment order of k_slice_rows.  Workgroup: 4 waves, tile 64 x 128 (4 row
+// and 8 column blocks of 16); wave: 32 x 64 as 2 x 4 MFMA 16x16x32 blocks.
+// The 48 fragments of a 32-k step (4 + 8 blocks x 4 slices, 1 KB each) are
+// copied global -> LDS by the four waves (global_load_lds, 16 bytes per
+// lane, no registers), double-buffered: step s + 1 loads while step s
+// multiplies.  Lane l feeds row (column) l % 16 of a block and the eight k
+// at 8 (l / 16) of the step; it holds rows 4 (l / 16) + v, column l % 16 of
+// the 16 x 16 result.  Every load is inside the padded planes; only the
+// stores are bounded by m x n.
+constexpr int SPL_NF = (SPL_BM / 16 + SPL_BN / 16) * SPL_S;   // fragments per k step
+
+__device__ static inline void stage_step(bf16x8 (*st)[64], const bf16x8 *__restrict__ A,
+                                         const bf16x8 *__restrict__ B, size_t pa, size_t pb, long long rb0,
+                                         long long cb0, long long nks, long long kk, int w, int lane) {
+#pragma unroll
+  for (int q = 0; q < SPL_NF / 4; ++q) {
+    const int f = w * (SPL_NF / 4) + q;
+    const bf16x8 *src;
+    if (f < (SPL_BM / 16) * SPL_S) {
+      const int rb = f / SPL_S, t = f % SPL_S;
+      src = A + t * pa + ((size_t)(rb0 + rb) * nks + kk) * 64 + lane;
+    } else {
+      const int g = f - (SPL_BM / 16) * SPL_S, cb = g / SPL_S, t = g % SPL_S;
+      src = B + t * pb + ((size_t)(cb0 + cb) * nks + kk) * 64 + lane;
+    }
+    __builtin_amdgcn_global_load_lds((const void *)src, (void __attribute__((address_space(3))) *)(&st[f][0]), 16, 0,
+                                     0);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_split_gemm(const bf16x8 *__restrict__ A, const bf16x8 *__restrict__ B,
+                                                    const int *__restrict__ ea, const int *__restrict__ eb,
+                                                    long long mpad, long long npad, long long kpad, long long m,
+                                                    long long n, double alpha, double beta, double *__restrict__ C,
+                                                    long long ldc) {
+  __shared__ bf16x8 lds[2][SPL_NF][64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int lr = lane & 15, lg = lane >> 4;
+  const long long rb0 = (long long)blockIdx.x * (SPL_BM / 16), cb0 = (long long)blockIdx.y * (SPL_BN / 16);
+  const int wr = (w & 1) * 2, wc = (w >> 1) * 4;   // the wave's first row / column block in the tile
+  const long long nks = kpad / 32;
+  const size_t pa = (size_t)(mpad / 16) * nks * 64, pb = (size_t)(npad / 16) * nks * 64;
+  double acc[2][4][4];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) acc[i][j][v] = 0.0;
+  stage_step(lds[0], A, B, pa, pb, rb0, cb0, nks, 0, w, lane);
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  for (long long kc = 0; kc < nks; kc += SPL_KC / 32) {
+    f32x4 l2[2][4], l3[2][4], l4[2][4], l5[2][4];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) l2[i][j] = l3[i][j] = l4[i][j] = l5[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int ks = 0; ks < SPL_KC / 32; ++ks) {
+      const long long kk = kc + ks;
+      const int cur = (int)(kk & 1);
+      if (kk + 1 < nks) stage_step(lds[cur ^ 1], A, B, pa, pb, rb0, cb0, nks, kk + 1, w, lane);
+      bf16x8 a[2][SPL_S], b[4][SPL_S];
+#pragma unroll
+      for (int t = 0; t < SPL_S; ++t) {
+#pragma unroll
+        for (int i = 0; i < 2; ++i) a[i][t] = lds[cur][(wr + i) * SPL_S + t][lane];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) b[j][t] = lds[cur][(SPL_BM / 16 + wc + j) * SPL_S + t][lane];
+      }
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          l2[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i][0], b[j][0], l2[i][j], 0, 0, 0);
+          l3[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i][0], b[j][1], l3[i][j], 0, 0, 0);
+          l3[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i][1], b[j][0], l3[i][j], 0, 0, 0);
+          l4[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i][0], b[j][2], l4[i][j], 0, 0, 0);
+          l4[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i][1], b[j][1], l4[i][j], 0, 0, 0);
+          l4[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i][2], b[j][0], l4[i][j], 0, 0, 0);
+          l5[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i][0], b[j][3], l5[i][j], 0, 0, 0);
+          l5[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i][1], b[j][2], l5[i][j], 0, 0, 0);
+          l5[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i][2], b[j][1], l5[i][j], 0, 0, 0);
+          l5[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i][3], b[j][0], l5[i][j], 0, 0, 0);
+        }
+      // the next step's fragments have landed and every wave is done with
+      // this step's buffer before it is refilled
+      __builtin_amdgcn_s_waitcnt(0);
+      __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+          acc[i][j][v] += (double)l2[i][j][v] * 0x1p-16 + (double)l3[i][j][v] * 0x1p-24 +
+                          (double)l4[i][j][v] * 0x1p-32 + (double)l5[i][j][v] * 0x1p-40;
+  }
+  const long long row0 = rb0 * 16 + wr * 16, col0 = cb0 * 16 + wc * 16;
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const long long r = row0 + 16 * i + 4 * lg + v, c = col0 + 16 * j + lr;
+        if (r < m && c < n) {
+          const double val = alpha * ldexp(acc[i][j][v], ea[r] + eb[c]);
+          double *p = C + r + ldc * c;
+          *p = beta == 0.0 ? val : beta * *p + val;
+        }
+      }
+}
+
+static long long round_up(long long x, long long q) { return (x + q - 1) / q * q; }
+
+// Slice the rows of one operand (element (r, k) at X[r rs + k ks]) into the
+// context's workspace `which`.
+static const bf16x8 *slice_operand(Context &C, int which, const double *X, long long rs, long long ks, long long rows,
+                                   long long kd, long long rows_pad, long long kpad, const int **e) {
+  DBuf<unsigned long long> &rmax = C.spl_max[which];
+  DBuf<unsigned short> &P = C.spl_slices[which];
+  DBuf<int> &ex = C.spl_exp[which];
+  rmax.alloc((size_t)rows);
+  P.alloc((size_t)SPL_S * (size_t)rows_pad * (size_t)kpad);
+  ex.alloc((size_t)rows_pad);
+  GI_HIP(hipMemsetAsync(rmax.p, 0, sizeof(unsigned long long) * (size_t)rows, C.stream));
+  dim3 g1((unsigned)((rows + SPL_T - 1) / SPL_T), (unsigned)((kd + SPL_T - 1) / SPL_T));
+  hipLaunchKernelGGL(k_row_absmax, g1, dim3(256), 0, C.stream, X, rs, ks, rows, kd, rmax.p);
+  dim3 g2((unsigned)(rows_pad / SPL_T), (unsigned)(kpad / SPL_T));
+  hipLaunchKernelGGL(k_slice_rows, g2, dim3(256), 0, C.stream, X, rs, ks, rows, kd, (const unsigned long long *)rmax.p,
+                     rows_pad, kpad, (bf16x8 *)P.p, ex.p);
+  GI_HIP(hipGetLastError());
+  *e = ex.p;
+  return (const bf16x8 *)P.p;
 }
 
 // C (m x n) = beta C + alpha op(A) op(B), op as in rocblas (none / transpose);
@@ -68,41 +263,22 @@ void split_bf16_gemm(Context &C, rocblas_operation ta, rocblas_operation tb, lon
                      double alpha, const double *A, long long lda, const double *B, long long ldb, double beta,
                      double *Cd, long long ldc) {
   if (m <= 0 || n <= 0) return;
-  const long long ra = ta == rocblas_operation_none ? m : k, ca = ta == rocblas_operation_none ? k : m;
-  const long long rb = tb == rocblas_operation_none ? k : n, cb = tb == rocblas_operation_none ? n : k;
   if (k <= 0) {
     if (beta != 1.0) throw Error{63, "split product with k = 0 and beta != 1"};
     return;
   }
-  DBuf<__hip_bfloat16> a1, a2, a3, b1, b2, b3;
-  DBuf<float> P, Q;
-  a1.alloc((size_t)(ra * ca));
-  a2.alloc((size_t)(ra * ca));
-  a3.alloc((size_t)(ra * ca));
-  b1.alloc((size_t)(rb * cb));
-  b2.alloc((size_t)(rb * cb));
-  b3.alloc((size_t)(rb * cb));
-  P.alloc((size_t)(m * n));
-  Q.alloc((size_t)(m * n));
-  hipLaunchKernelGGL(k_split3, dim3(grid_for(ra * ca)), dim3(256), 0, C.stream, A, lda, ra, ca, a1.p, a2.p, a3.p);
-  hipLaunchKernelGGL(k_split3, dim3(grid_for(rb * cb)), dim3(256), 0, C.stream, B, ldb, rb, cb, b1.p, b2.p, b3.p);
-  GI_HIP(hipGetLastError());
-  const float one = 1.0f, zero = 0.0f;
-  auto prod = [&](const __hip_bfloat16 *x, const __hip_bfloat16 *y, float *out, const float *bt) {
-    SPLIT_RB(rocblas_gemm_ex(C.blas, ta, tb, (rocblas_int)m, (rocblas_int)n, (rocblas_int)k, &one, x,
-                             rocblas_datatype_bf16_r, (rocblas_int)ra, y, rocblas_datatype_bf16_r, (rocblas_int)rb, bt,
-                             out, rocblas_datatype_f32_r, (rocblas_int)m, out, rocblas_datatype_f32_r, (rocblas_int)m,
-                             rocblas_datatype_f32_r, rocblas_gemm_algo_standard, 0, 0));
-  };
-  // corrections, smallest first, then the leading product
-  prod(a3.p, b1.p, Q.p, &zero);
-  prod(a2.p, b2.p, Q.p, &one);
-  prod(a1.p, b3.p, Q.p, &one);
-  prod(a2.p, b1.p, Q.p, &one);
-  prod(a1.p, b2.p, Q.p, &one);
-  prod(a1.p, b1.p, P.p, &zero);
-  hipLaunchKernelGGL(k_add_partials, dim3(grid_for(m * n)), dim3(256), 0, C.stream, Cd, ldc, m, n, alpha, beta, P.p,
-                     Q.p);
+  const long long kpad = round_up(k, SPL_KC), mpad = round_up(m, SPL_BM), npad = round_up(n, SPL_BN);
+  if (kpad / SPL_T > 65535 || npad / SPL_BN > 65535)
+    throw Error{63, "split product: dimension beyond the launch grid"};
+  // row r of op(A): op(A)(r, k); row c of op(B)^T: op(B)(k, c)
+  const long long ars = ta == rocblas_operation_none ? 1 : lda, aks = ta == rocblas_operation_none ? lda : 1;
+  const long long brs = tb == rocblas_operation_none ? ldb : 1, bks = tb == rocblas_operation_none ? 1 : ldb;
+  const int *ea = nullptr, *eb = nullptr;
+  const bf16x8 *pa = slice_operand(C, 0, A, ars, aks, m, k, mpad, kpad, &ea);
+  const bf16x8 *pb = slice_operand(C, 1, B, brs, bks, n, k, npad, kpad, &eb);
+  dim3 grid((unsigned)(mpad / SPL_BM), (unsigned)(npad / SPL_BN));
+  hipLaunchKernelGGL(k_split_gemm, grid, dim3(256), 0, C.stream, pa, pb, ea, eb, mpad, npad, kpad, m, n, alpha, beta,
+                     Cd, ldc);
   GI_HIP(hipGetLastError());
 }
 

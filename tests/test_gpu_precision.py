@@ -1,7 +1,8 @@
 """Reconstruction error of the decomposition (gi_chol_residual) and the
-precision form: split-BF16 products with FP32 accumulation for the MO
-transform (GI_MO_SPLIT_BF16) and the Step-1 trailing update
-(GI_CHOL_SPLIT_BF16), against the FP64 path."""
+precision form: split-BF16 products (four 8-bit slices per element, exact
+FP32 sums per 128 k, FP64 across chunks) for the MO transform
+(GI_MO_SPLIT_BF16) and the Step-1 trailing update (GI_CHOL_SPLIT_BF16),
+against the FP64 path."""
 import numpy as np
 import pytest
 
@@ -74,9 +75,13 @@ def test_split_mo_transform_error(medium):
     nocc, nvirt = 10, 60
     r64 = ctx.mo_transform(Cm, 1, nocc, Cm, nocc + 1, nocc + nvirt)
     r16 = ctx.mo_transform(Cm, 1, nocc, Cm, nocc + 1, nocc + nvirt, flags=lib.MO_SPLIT_BF16)
-    # FP32 accumulation over NAO terms of |W||C|: measured 8e-8 relative
-    assert np.abs(r16 - r64).max() <= 1e-6 * np.abs(r64).max()
-    assert np.abs(r16 - r64).max() > 0.0          # it is the split path
+    # four 8-bit slices per element, exact FP32 sums per 128 k: the error is
+    # the slice truncation, ~2^-32 (max|W| sum|C| + ...) per element;
+    # measured 1.8e-9 of max |R| (FP32 accumulation gave 8e-8)
+    err = np.abs(r16 - r64).max()
+    print("split MO transform: max abs err %.3e (max |R| %.3e)" % (err, np.abs(r64).max()))
+    assert err <= 1e-8 * np.abs(r64).max()
+    assert err > 0.0                              # it is the split path
 
 
 def test_split_trailing_update_keeps_a_valid_decomposition(medium):
@@ -87,9 +92,11 @@ def test_split_trailing_update_keeps_a_valid_decomposition(medium):
     meta = ctx.chol_meta()
     piv = meta["pivot_order"]
     assert len(np.unique(piv)) == len(piv) == meta["nvecs"]
-    assert abs(meta["nvecs"] - nv64) <= 0.05 * nv64
     mx, mn = ctx.chol_residual()
-    # the split update changes the pivots; the result is still a
-    # decomposition of V (PSD residual), with a residual of the order of tau
+    print("split trailing update: nvecs %d (FP64 %d), residual max %.3e min %.3e" % (meta["nvecs"], nv64, mx, mn))
+    # the split update perturbs the residual diagonals by ~1e-9, far below
+    # tau: the decomposition keeps (nearly) the FP64 pivot count and stays
+    # within tau with a PSD residual
+    assert abs(meta["nvecs"] - nv64) <= max(2, 0.002 * nv64)
     assert mn >= -1e-10
-    assert mx <= 100 * TAU
+    assert mx <= TAU * 1.01
