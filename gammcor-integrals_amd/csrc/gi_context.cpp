@@ -53,7 +53,6 @@ Context::~Context() {
   chol.mo_adj_p.release();
   for (int i = 0; i < 2; ++i) {
     spl_slices[i].release();
-    spl_max[i].release();
     spl_exp[i].release();
   }
   // every buffer is released (stream-ordered frees on `stream`) before the

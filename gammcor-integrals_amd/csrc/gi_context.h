@@ -119,10 +119,9 @@ struct Context {
     md_et_used = md_prim_used = 0;
   }
   CholState chol;
-  // split-BF16 products (split.hip): BF16 slice planes, row maxima and row
+  // split-BF16 products (split.hip): BF16 slice planes and per-chunk row
   // scale exponents of the two operands; grow-only, kept across calls
   DBuf<unsigned short> spl_slices[2];
-  DBuf<unsigned long long> spl_max[2];
   DBuf<int> spl_exp[2];
   // profiling of ERI launches (events on the library stream)
   bool time_eri = false;
