@@ -3,7 +3,7 @@
 //
 // Error-free splitting (Ozaki, Ogita, Oishi, Rump, Numer. Algorithms 59
 // (2012) 95-118) onto BF16 MFMA with FP32 accumulation, k taken in chunks of
-// 128:
+// 256:
 //  * within a chunk, row r of op(A) is scaled by 2^-e and column c of op(B)
 //    by 2^-f, the powers of two that bring the largest element of the row
 //    (column) in that chunk into [1/2, 1);
@@ -13,10 +13,12 @@
 //  * A B is the sum over slice pairs (t, u) with t + u <= 5 (ten products,
 //    the omitted pairs are below 2^-33) of 2^(-8(t+u)) q_t p_u; the pairs of
 //    one level t + u share an FP32 accumulator;
-//  * q_t p_u is an integer below 2^16 in magnitude, so every partial sum of a
-//    level over the 128 k of a chunk is an integer below 2^24: exact in FP32
-//    whatever the order of the MFMA's additions.  At the end of a chunk the
-//    four level sums are scaled by 2^(e + f - 8 level) and added into an FP64
+//  * the leading levels carry at most 2^16 per k (q_1 p_1, and q_1 p_2 +
+//    q_2 p_1), so their partial sums over the 256 k of a chunk are integers
+//    of at most 2^24: exact in FP32 whatever the order of the MFMA's
+//    additions.  Levels 4 and 5 are below 2^-17 of the product scale; their
+//    FP32 rounding (2^-24 of that) stays under 2^-41.  At the end of a chunk
+//    the level sums are scaled by 2^(e + f - 8 level) and added into an FP64
 //    accumulator.
 // The only error is the truncation of the slices: per chunk
 //   |err(r, c)| <~ 2^-32 sum_k (max|A(r,chunk)| |B(k,c)| + |A(r,k)| max|B(chunk,c)|),
@@ -30,10 +32,10 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int SPL_S = 4;      // slices per element
-constexpr int SPL_KC = 128;   // k per exact FP32 chunk; k is padded to a multiple
+constexpr int SPL_KC = 256;   // k per FP32 chunk; k is padded to a multiple
 constexpr int SPL_BM = 64;    // workgroup tile rows: 2 waves x 32
 constexpr int SPL_BN = 128;   // workgroup tile columns: 2 waves x 64
-constexpr int SPL_T = 64;     // slicing tile: 64 rows x one chunk of k
+constexpr int SPL_T = 32;     // slicing tile: 32 rows x one chunk of k
 
 // Slices of the rows r < rows_pad, k < kpad (zero outside rows x kd; element
 // (r, k) at X[r rs + k ks], one of rs, ks is 1) in the MFMA fragment order:
@@ -42,7 +44,7 @@ constexpr int SPL_T = 64;     // slicing tile: 64 rows x one chunk of k
 //   l = r % 16 + 16 ((k % 32) / 8),
 // so one wave reads the 16 x 32 fragment (rb, ks) of slice t as one
 // contiguous kilobyte.  e[ci rows_pad + r] = the scale exponent of row r in
-// chunk ci.  One workgroup per 64 rows x 128 k tile, read along the
+// chunk ci.  One workgroup per 32 rows x 256 k tile, read along the
 // contiguous index into LDS.
 __global__ __launch_bounds__(256) void k_slice_rows(const double *__restrict__ X, long long rs, long long ks,
                                                     long long rows, long long kd, long long rows_pad, long long kpad,
@@ -75,9 +77,9 @@ __global__ __launch_bounds__(256) void k_slice_rows(const double *__restrict__ X
   __syncthreads();
   const long long nrb = rows_pad / 16, nks = kpad / 32;
   const size_t plane = (size_t)nrb * (size_t)nks * 64;
-  // 4 row blocks x 4 k steps x 64 lanes of 8 k each in the tile
-  for (int i = threadIdx.x; i < 1024; i += blockDim.x) {
-    const int l = i & 63, kb = (i >> 6) & 3, rbl = i >> 8;
+  // 2 row blocks x 8 k steps x 64 lanes of 8 k each in the tile
+  for (int i = threadIdx.x; i < (SPL_T / 16) * (SPL_KC / 32) * 64; i += blockDim.x) {
+    const int l = i & 63, kb = (i >> 6) % (SPL_KC / 32), rbl = (i >> 6) / (SPL_KC / 32);
     const int ri = 16 * rbl + (l & 15), kq = 32 * kb + 8 * (l >> 4);
     const size_t u = ((size_t)(r0 / 16 + rbl) * (size_t)nks + (size_t)(k0 / 32 + kb)) * 64 + (size_t)l;
     double a[8];
@@ -220,7 +222,7 @@ __global__ __launch_bounds__(256) void k_split_gemm(const bf16x8 *__restrict__ A
 #pragma unroll
         for (int v = 0; v < 4; ++v)
           acc[i][j][v] += ldexp((double)l2[i][j][v] * 0x1p-16 + (double)l3[i][j][v] * 0x1p-24 +
-                                    (double)l4[i][j][v] * 0x1p-32 + (double)l5[i][j][v] * 0x1p-40,
+                                    (double)(l4[i][j][v] + l5[i][j][v] * 0x1p-8f) * 0x1p-32,
                                 er[i][v] + ec[j]);
   }
 #pragma unroll
