@@ -124,7 +124,8 @@ def batched_form(nworkers: int, autotune_batch: bool = True):
 
 
 def precision_form(ctx, basis, Cmo, hip, reps: int = 3):
-    """Split-BF16 products with FP32 accumulation against the FP64 path on the
+    """Split-BF16 MFMA products (four 8-bit slices per element, exact FP32
+    sums per 256 k, FP64 across chunks) against the FP64 path on the
     same (H2O)16 system (1200 AOs, the nbf of the precision configuration):
     the AO->MO transform with nocc=80, nvirt=600 (device timings, max error
     over all Rkab elements) and the Step-1 trailing update (build time,

@@ -40,16 +40,19 @@ typedef struct gi_context gi_context;
                                    keys quantized to 2^-26 relative (BASE sort) and
                                    2^-32 of max(Vdiag) (residual diagonals) */
 #define GI_CHOL_SPLIT_BF16 16   /* precision form: the Step-1 trailing update
-                                   (chol2_SubtractR) as split-BF16 products with
-                                   FP32 accumulation instead of FP64 dgemm */
+                                   (chol2_SubtractR) as split-BF16 MFMA products
+                                   (four 8-bit slices per element, exact FP32
+                                   sums per 256 k, FP64 across chunks; error
+                                   ~2^-33 of the operand scales) instead of FP64
+                                   dgemm */
 
 /* gi_mo_transform flags */
 #define GI_MO_ALL_COLUMNS 1     /* transform every a column even when a0 > 1
                                    (the reference's pass bounds,
                                    Cholesky_Gammcor.f90:196-197, skip some) */
 #define GI_MO_SPLIT_BF16 2      /* precision form: the second transform step
-                                   (Rab = W CB) as split-BF16 products with FP32
-                                   accumulation instead of FP64 dgemm */
+                                   (Rab = W CB) as split-BF16 MFMA products (as
+                                   GI_CHOL_SPLIT_BF16) instead of FP64 dgemm */
 
 /* Create a context on HIP device `device` (no reference counterpart: the
  * reference initialises module state in chol_gammcor_Rkpq,
