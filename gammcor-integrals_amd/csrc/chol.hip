@@ -104,6 +104,30 @@ struct MicroState {
   double val;     // raw residual diagonal of the current pivot
 };
 
+// Device work arrays of gi_chol_build, kept in the context across builds and
+// grown only: a repeated build (same or smaller system) neither allocates nor
+// frees device memory.  Stream-ordered frees of multi-GB blocks between the
+// macro iterations stalled the host for up to 0.7 s in long runs.
+struct CholWork {
+  DBuf<double> D, Dn, L, Ln, M, LB, tmp, bm, Mb, Lb, Db, LP;   // Step 1
+  DBuf<int> piv;
+  DBuf<MicroState> state;
+  DBuf<long long> src, loc, bd;
+  DBuf<unsigned char> excl;
+  DBuf<int> dim;
+  DBuf<int> info;            // factorization of V(P, P)
+  DBuf<double> X, W;
+  DBuf<double> Wv;           // Vabrs staging of the full-dimension vectors
+};
+static CholWork &chol_work(Context &C) {
+  if (!C.cw) C.cw = new CholWork();
+  return *C.cw;
+}
+void release_chol_work(Context &C) {
+  delete C.cw;
+  C.cw = nullptr;
+}
+
 
 __device__ __forceinline__ void argmax_merge(double &k1, double &v1, long long &i1, double k2, double v2,
                                              long long i2) {
@@ -482,10 +506,11 @@ static void chol_inv_rec(Context &C, double *V, double *X, long long n, long lon
 // V (n x n, lower triangle significant) -> L^-1; throws the reference's error
 // when V is not positive definite
 static void factor_and_invert(Context &C, DBuf<double> &V, long long n) {
-  DBuf<int> dinfo;
+  CholWork &cw = chol_work(C);
+  DBuf<int> &dinfo = cw.info;
   dinfo.alloc(1);
   hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, C.stream, dinfo.p, 0);
-  DBuf<double> X, W;
+  DBuf<double> &X = cw.X, &W = cw.W;
   X.alloc((size_t)n * n);
   GI_HIP(hipMemsetAsync(X.p, 0, sizeof(double) * (size_t)n * n, C.stream));
   const long long h = n / 2 + POTF_NB;
@@ -735,9 +760,11 @@ void chol_build(Context &C, double tau, double kappa, long long max_block_dim, l
     long long nsps_cand = nsp, nops_cand = nop;
     std::vector<long long> cand(nsp + 1);   // candidate shell pairs, compressed order
     for (long long k = 1; k <= nsp; ++k) { cand[k] = k; loc_comp[k] = loc_full[k]; }
-    DBuf<double> dD, dDn, dL, dLn, dM, dLB, dtmp, dbm, dMb, dLb, dDb, dLP;
-    DBuf<int> dpiv;
-    DBuf<MicroState> dstate;
+    CholWork &cw = chol_work(C);
+    DBuf<double> &dD = cw.D, &dDn = cw.Dn, &dL = cw.L, &dLn = cw.Ln, &dM = cw.M, &dLB = cw.LB, &dtmp = cw.tmp,
+                 &dbm = cw.bm, &dMb = cw.Mb, &dLb = cw.Lb, &dDb = cw.Db, &dLP = cw.LP;
+    DBuf<int> &dpiv = cw.piv;
+    DBuf<MicroState> &dstate = cw.state;
     dD.upload(Dh.data(), nop, C.stream);
     long long ncand = nop;       // rows of L / length of D
     long long capL = 0;          // columns allocated in L
@@ -745,9 +772,9 @@ void chol_build(Context &C, double tau, double kappa, long long max_block_dim, l
     dmax = -INFINITY;
     for (double v : Dh) dmax = std::max(dmax, v);
     bool do_macro = max_nchol > 0;
-    DBuf<long long> dsrc, dloc, dbd;
-    DBuf<unsigned char> dexcl;
-    DBuf<int> ddim;
+    DBuf<long long> &dsrc = cw.src, &dloc = cw.loc, &dbd = cw.bd;
+    DBuf<unsigned char> &dexcl = cw.excl;
+    DBuf<int> &ddim = cw.dim;
     std::vector<double> mv;
     std::vector<long long> im;
     while (do_macro) {
@@ -1091,7 +1118,7 @@ void chol_build(Context &C, double tau, double kappa, long long max_block_dim, l
     // subsets per group: keep the Vabrs staging under ~16 GiB
     long long per_group = std::max<long long>(1, (long long)((16.0 * (1ull << 30)) / (8.0 * (double)std::max<size_t>(slab, 1))));
     per_group = std::min(per_group, nsub);
-    DBuf<double> dW;
+    DBuf<double> &dW = chol_work(C).Wv;
     dW.alloc(slab * per_group);
     double t_vab = 0, t_gemm = 0;
     for (long long X0 = 1; X0 <= nsub; X0 += per_group) {

@@ -86,6 +86,8 @@ struct CholState {
   long long eri_flops = 0;
 };
 
+struct CholWork;   // gi_chol_build work arrays (chol.hip)
+
 struct Context {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -119,6 +121,8 @@ struct Context {
     md_et_used = md_prim_used = 0;
   }
   CholState chol;
+  // device work arrays of gi_chol_build (chol.hip), grow-only across builds
+  CholWork *cw = nullptr;
   // split-BF16 products (split.hip): BF16 slice planes and per-chunk row
   // scale exponents of the two operands; grow-only, kept across calls
   DBuf<unsigned short> spl_slices[2];
@@ -165,6 +169,9 @@ void upload_constants(Context &C);
 void split_bf16_gemm(Context &C, rocblas_operation ta, rocblas_operation tb, long long m, long long n, long long k,
                      double alpha, const double *A, long long lda, const double *B, long long ldb, double beta,
                      double *Cd, long long ldc);
+
+// frees the context's gi_chol_build work arrays (chol.hip)
+void release_chol_work(Context &C);
 
 // max and min over BASE orbital pairs of (pq|pq) - sum_k Rkpq(k, pq)^2 (chol.hip)
 void chol_residual(Context &C, double *rmax, double *rmin);
