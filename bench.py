@@ -311,10 +311,17 @@ def main():
     # algorithmic FP64 ops per launch / average launch duration (= total / total)
     achieved = eri_flops / eri_s / 1e12 if eri_s > 0 else None
     # Rkpq = Inv_L Vabrs^T with Inv_L lower triangular, one GEMM per block row
+    # of a multiple of 128 rows counted from the last row (the library's
+    # partition for the default vec_blocks = 8)
     nv = d["nvecs"]
     nblk = max(1, min(8, nv // 512))
-    gemm_flops = sum(2.0 * (nv * (rb + 1) // nblk - nv * rb // nblk) * (nv * (rb + 1) // nblk)
-                     for rb in range(nblk)) * d["norb_pairs"]
+    rows_blk = max(128, (nv + 64 * nblk) // (128 * nblk) * 128)
+    gemm_flops, r1 = 0.0, nv
+    while r1 > 0:
+        r0 = max(0, r1 - rows_blk)
+        gemm_flops += 2.0 * (r1 - r0) * r1
+        r1 = r0
+    gemm_flops *= d["norb_pairs"]
     out = {
         "metric": METRIC,
         "value": quartets / t_build,

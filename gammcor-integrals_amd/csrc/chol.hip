@@ -1161,13 +1161,17 @@ void chol_build(Context &C, double tau, double kappa, long long max_block_dim, l
       // block row r only needs the first r1 columns of Inv_L, so the product
       // is one GEMM per block row ((nblk + 1) / (2 nblk) of the dense FLOPs)
       const double one = 1.0, zero = 0.0;
+      // Block rows hold a multiple of 128 rows (the DGEMM macro tile), counted
+      // from the last row: only the first block, whose product is the
+      // shortest (k = r1), has a partly filled tile row.
       const long long nblk = std::max<long long>(1, std::min<long long>(C.vec_blocks, nvec / 512));
-      for (long long rb = 0; rb < nblk; ++rb) {
-        const long long r0 = nvec * rb / nblk, r1 = nvec * (rb + 1) / nblk;
-        if (r1 <= r0) continue;
+      const long long rows_blk = std::max<long long>(128, (nvec + 64 * nblk) / (128 * nblk) * 128);
+      for (long long r1 = nvec; r1 > 0;) {
+        const long long r0 = std::max<long long>(0, r1 - rows_blk);
         ROCBLAS_CHECK(rocblas_dgemm(C.blas, rocblas_operation_none, rocblas_operation_transpose, r1 - r0, ldw, r1,
                                     &one, S.inv_l.p + r0, nvec, dW.p, ldw, &zero,
                                     S.rkpq.p + (size_t)(X0 - 1) * nvec * msd + r0, nvec));
+        r1 = r0;
       }
       GI_HIP(hipStreamSynchronize(C.stream));
       t_gemm += now() - tg;
