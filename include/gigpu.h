@@ -56,7 +56,10 @@ typedef struct gi_context gi_context;
 
 /* Create a context on HIP device `device` (no reference counterpart: the
  * reference initialises module state in chol_gammcor_Rkpq,
- * Cholesky_Gammcor.f90:347-352). */
+ * Cholesky_Gammcor.f90:347-352).  A context keeps its device work arrays
+ * (the Cholesky panels, the vector staging, the factorization scratch and
+ * the split-product slices) across calls, grown only, so that repeated
+ * builds allocate nothing; gi_destroy frees them. */
 int gi_create(gi_context **ctx, int64_t device);
 int gi_destroy(gi_context *ctx);
 /* Message of the last failure on this thread (replaces msg(...,MSG_ERROR) +
