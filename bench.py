@@ -312,9 +312,9 @@ def main():
     achieved = eri_flops / eri_s / 1e12 if eri_s > 0 else None
     # Rkpq = Inv_L Vabrs^T with Inv_L lower triangular, one GEMM per block row
     # of a multiple of 128 rows counted from the last row (the library's
-    # partition for the default vec_blocks = 8)
+    # partition for the default vec_blocks = 20)
     nv = d["nvecs"]
-    nblk = max(1, min(8, nv // 512))
+    nblk = max(1, min(20, nv // 256))
     rows_blk = max(128, (nv + 64 * nblk) // (128 * nblk) * 128)
     gemm_flops, r1 = 0.0, nv
     while r1 > 0:

@@ -1164,7 +1164,7 @@ void chol_build(Context &C, double tau, double kappa, long long max_block_dim, l
       // Block rows hold a multiple of 128 rows (the DGEMM macro tile), counted
       // from the last row: only the first block, whose product is the
       // shortest (k = r1), has a partly filled tile row.
-      const long long nblk = std::max<long long>(1, std::min<long long>(C.vec_blocks, nvec / 512));
+      const long long nblk = std::max<long long>(1, std::min<long long>(C.vec_blocks, nvec / 256));
       const long long rows_blk = std::max<long long>(128, (nvec + 64 * nblk) / (128 * nblk) * 128);
       for (long long r1 = nvec; r1 > 0;) {
         const long long r0 = std::max<long long>(0, r1 - rows_blk);

@@ -140,7 +140,7 @@ struct Context {
   long long eri_min_tiles = 8192, eri_max_kt = 32;
   bool eri_lane = true, eri_trans = true;
   long long eri_lane_emax = LANE_EMAX;   // lane kernels only for classes with E, F <= this
-  long long vec_blocks = 8;              // block rows of the triangular Inv_L Vabrs^T product
+  long long vec_blocks = 20;             // block rows of the triangular Inv_L Vabrs^T product
 
   Context(int dev);
   ~Context();

@@ -23,7 +23,7 @@ CANDIDATES = {
     "eri_max_kt": [32, 16, 64],
     "eri_min_tiles": [8192, 4096, 16384],
     "eri_lane_emax": [4, 2],
-    "vec_blocks": [8, 6, 10],
+    "vec_blocks": [20, 12, 30],
 }
 DEFAULTS = {k: v[0] for k, v in CANDIDATES.items()}
 

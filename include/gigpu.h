@@ -104,7 +104,8 @@ int gi_eri_pair_block(gi_context *ctx, int64_t nrow_pairs, const int64_t *row_pa
  *                    on the 16-row tiles, 0: never transpose (1)
  *   "eri_lane_emax"  lane kernels only for classes with E, F <= this (4)
  *   "vec_blocks"     block rows of the triangular Inv_L Vabrs^T product of
- *                    the full-dimension vectors (8)
+ *                    the full-dimension vectors (20; at most NVecs / 256, each
+ *                    a multiple of 128 rows counted from the last row)
  * Unknown names fail.  Results do not depend on these settings beyond
  * floating-point rounding. */
 int gi_set_option(gi_context *ctx, const char *name, int64_t value);
