@@ -590,8 +590,9 @@ void chol_build(Context &C, double tau, double kappa, long long max_block_dim, l
   if (max_block_dim <= 0 || block_dim <= 0) throw Error{5, "MaxBlockDim and CholVecsBlock must be positive"};
   CholState &S = C.chol;
   S.valid = false;
-  S.inv_l.release();
-  S.rkpq.release();
+  // Inv_L and Rkpq keep their device arrays across builds (grow-only); a
+  // build without vectors must not leave an earlier build's Rkpq readable
+  if (flags & 2) S.rkpq.release();
   S.mo_adj_off.release();
   S.mo_adj_col.release();
   S.mo_adj_p.release();
@@ -828,6 +829,7 @@ void chol_build(Context &C, double tau, double kappa, long long max_block_dim, l
                            dDn.p, ncomp, dD.p, ncand, dsrc.p, ncomp, 1LL);
         std::swap(dD.p, dDn.p);
         std::swap(dD.n, dDn.n);
+        std::swap(dD.st, dDn.st);
         // new L with room for this macro iteration's vectors
         long long mnb = std::min(ncomp, max_nbatch_cap);
         long long newcap = std::min(max_nchol, nchol + mnb);
@@ -840,7 +842,7 @@ void chol_build(Context &C, double tau, double kappa, long long max_block_dim, l
         }
         std::swap(dL.p, dLn.p);
         std::swap(dL.n, dLn.n);
-        dLn.release();
+        std::swap(dL.st, dLn.st);
         capL = newcap;
         for (long long g = 1; g <= nsps_cand; ++g) loc_comp[cand[g]] = newloc[g];
         ncand = ncomp;
