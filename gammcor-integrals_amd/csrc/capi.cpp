@@ -458,4 +458,26 @@ int gi_mo_transform(gi_context *cp, const double *CA, int64_t lda, int64_t a0, i
   })
 }
 
+int gi_ao_map(gi_context *cp, int64_t ordering, int64_t *map, double *phase) {
+  GI_TRY({
+    gi::Context &C = ctx_of(cp);
+    std::vector<long long> m;
+    std::vector<double> ph;
+    gi::ao_map(C, (int)ordering, m, ph);
+    for (int p = 0; p < C.nao; ++p) {
+      map[p] = (int64_t)m[p] + 1;
+      phase[p] = ph[p];
+    }
+  })
+}
+
+int gi_ao_rows_to_auto2e_device(gi_context *cp, int64_t ordering, const double *X_ext, int64_t ld_ext,
+                                int64_t ncols, double *X_ao, int64_t ld_ao) {
+  GI_TRY({
+    gi::Context &C = ctx_of(cp);
+    if (X_ext == X_ao) throw Error{72, "in-place AO reordering is not supported"};
+    gi::ao_rows_to_auto2e(C, (int)ordering, X_ext, ld_ext, ncols, X_ao, ld_ao);
+  })
+}
+
 }  // extern "C"

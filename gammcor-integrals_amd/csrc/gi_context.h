@@ -170,6 +170,13 @@ void split_bf16_gemm(Context &C, rocblas_operation ta, rocblas_operation tb, lon
                      double alpha, const double *A, long long lda, const double *B, long long ldb, double beta,
                      double *Cd, long long ldc);
 
+// external AO order -> Auto2e order (aomap.hip): map[p] = Auto2e position of
+// external AO p (0-based), phase in Auto2e order (Orca signs, else 1)
+void ao_map(const Context &C, int ordering, std::vector<long long> &map, std::vector<double> &phase);
+// rows of X (nao x ncols, device) to Auto2e order with the phases applied
+void ao_rows_to_auto2e(Context &C, int ordering, const double *X, long long ldx, long long ncols, double *Y,
+                       long long ldy);
+
 // frees the context's gi_chol_build work arrays (chol.hip)
 void release_chol_work(Context &C);
 

@@ -26,7 +26,8 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH, "-Wno-pass-fai
           "-I" + CSRC, "-I" + os.path.join(os.path.dirname(PKG), "include")]
 LIBS = ["-L/opt/rocm/lib", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"]
 
-SOURCES = ["eri.hip", "chol.hip", "mo.hip", "grad.hip", "split.hip", "onee.hip", "gi_tables.cpp", "gi_context.cpp", "sort.cpp", "capi.cpp"]
+SOURCES = ["eri.hip", "chol.hip", "mo.hip", "grad.hip", "split.hip", "onee.hip", "aomap.hip", "gi_tables.cpp",
+           "gi_context.cpp", "sort.cpp", "capi.cpp"]
 HEADERS = ["gi_common.h", "gi_context.h", "md_kernel.h", "md_classes.inc", "grad_kernels.h", "onee_kernels.h"]
 
 

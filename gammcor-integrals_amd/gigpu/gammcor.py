@@ -125,16 +125,87 @@ def chol_gammcor_Rkpq(AOBasis, Accuracy: int, Omega: float | None = None, ctx: "
     return TCholeskyVecsOTF(Chol2Data=v, ctx=ctx, timings=ctx.chol_timings())
 
 
-def auto2e_interface_AngFuncTransf(X_in: np.ndarray, AOBasis, ExternalOrdering: int) -> np.ndarray:
-    """External-program AO order -> Auto2e order on the rows of X
-    (Auto2eInterface.f90:217-275).  For spherical AOs the Dalton and PySCF
-    orders equal the Auto2e order (s; px, py, pz; m = -l..l), so the map is
-    the identity (Auto2eInterface.f90:68-101)."""
-    if ExternalOrdering in (ORBITAL_ORDERING_AUTO2E,):
-        return X_in
-    if ExternalOrdering in (ORBITAL_ORDERING_DALTON, ORBITAL_ORDERING_PYSCF) and AOBasis.spher_ao:
-        return X_in
-    raise GammcorError("AO ordering %d is not supported by this build" % ExternalOrdering)
+def auto2e_interface_AOMap(AOBasis, ExternalOrdering: int, ctx):
+    """(Map_extao2ao, Map_ao2extao), 1-based (auto2e_interface_AOMap,
+    Auto2eInterface.f90:378-514), computed by the library for the basis set
+    on ``ctx`` (gi_ao_map).  Raises GammcorError("Invalid external orbital
+    ordering") where the reference stops."""
+    try:
+        m, _ = ctx.ao_map(ExternalOrdering)
+    except lib.GigpuError as e:
+        if "gigpu error 70:" in str(e):
+            raise GammcorError("Invalid external orbital ordering")
+        raise
+    inv = np.empty_like(m)
+    inv[m - 1] = np.arange(1, len(m) + 1)
+    return m, inv
+
+
+def auto2e_interface_AngFuncTransf(X_in: np.ndarray, AOBasis, ExternalOrdering: int, FromExternalAO: bool = True,
+                                   TwoIndexTransf: bool = False, ctx=None) -> np.ndarray:
+    """X_out from X_in between the external program's AO order and the Auto2e
+    order (Auto2eInterface.f90:217-275): FromExternalAO -> Map_extao2ao, else
+    Map_ao2extao; X_out(Map(p), :) = X_in(p, :), and with TwoIndexTransf also
+    the columns, X_out(Map(p), Map(q)) = X_in(p, q) (:278-310).  No signs: the
+    callers apply the Orca phases (auto2e_interface_ApplyOrcaPhases_Matrix)."""
+    ctx = _context_for(AOBasis, ctx)
+    ext2ao, ao2ext = auto2e_interface_AOMap(AOBasis, ExternalOrdering, ctx)
+    Map = (ext2ao if FromExternalAO else ao2ext) - 1
+    X_in = np.asarray(X_in, dtype=np.float64)
+    X_out = np.empty_like(X_in)
+    X_out[Map, ...] = X_in
+    if TwoIndexTransf:
+        W = X_out.copy()
+        X_out[:, Map] = W
+    return X_out
+
+
+def auto2e_interface_ApplyOrcaPhases_Matrix(X: np.ndarray, AOBasis, TwoIndexTransf: bool, ctx=None) -> np.ndarray:
+    """Orca's signs of the spherical functions with m = +-3 (f, g, h) and
+    m = +-4 (g, h) (Auto2eInterface.f90:517-585), applied in place to the
+    rows (and with TwoIndexTransf the columns) of X at their Auto2e
+    positions.  The reference's row loop runs over NAO columns (:574-577);
+    here every column of X is covered, which is the same for the square
+    matrices it is called with."""
+    ctx = _context_for(AOBasis, ctx)
+    _, phase = ctx.ao_map(ORBITAL_ORDERING_ORCA)
+    X *= phase[:, None]
+    if TwoIndexTransf:
+        X *= phase[None, :]
+    return X
+
+
+def auto2e_interface_C(C_extao: np.ndarray, AOBasis, ExternalOrdering: int, ctx=None) -> np.ndarray:
+    """C(external AO order) -> C(Auto2e order), Orca phases included
+    (auto2e_interface_C, Auto2eInterface.f90:313-337)."""
+    C_ao = auto2e_interface_AngFuncTransf(C_extao, AOBasis, ExternalOrdering, True, False, ctx)
+    if ExternalOrdering == ORBITAL_ORDERING_ORCA:
+        auto2e_interface_ApplyOrcaPhases_Matrix(C_ao, AOBasis, False, ctx)
+    return C_ao
+
+
+def auto2e_interface_V(V_extao: np.ndarray, AOBasis, ExternalOrdering: int, ctx=None) -> np.ndarray:
+    """V(external AO order, both indices) -> V(Auto2e order)
+    (auto2e_interface_V, Auto2eInterface.f90:340-361)."""
+    V_ao = auto2e_interface_AngFuncTransf(V_extao, AOBasis, ExternalOrdering, True, True, ctx)
+    if ExternalOrdering == ORBITAL_ORDERING_ORCA:
+        auto2e_interface_ApplyOrcaPhases_Matrix(V_ao, AOBasis, True, ctx)
+    return V_ao
+
+
+def _context_for(AOBasis, ctx):
+    """A library context holding AOBasis (the AO maps are computed there)."""
+    if ctx is not None and getattr(ctx, "basis", None) is AOBasis:
+        return ctx
+    global _map_ctx
+    if _map_ctx is None:
+        _map_ctx = lib.Context(0)
+    if getattr(_map_ctx, "basis", None) is not AOBasis:
+        _map_ctx.set_basis(AOBasis)
+    return _map_ctx
+
+
+_map_ctx = None
 
 
 def chol_gammcor_Rkab(Rkab, CA, a0, a1, CB, b0, b1, MaxBufferDimMB, CholeskyVecsOTF: TCholeskyVecsOTF, AOBasis,
@@ -151,8 +222,10 @@ def chol_gammcor_Rkab(Rkab, CA, a0, a1, CB, b0, b1, MaxBufferDimMB, CholeskyVecs
     nchol = CholeskyVecsOTF.Chol2Data.NVecs
     if Rkab is not None and (Rkab.shape[0] != nchol or Rkab.shape[1] != NA * NB):
         raise GammcorError("Invalid dimension of Rkab")
-    CA_ao = auto2e_interface_AngFuncTransf(CA, AOBasis, ExternalOrdering)
-    CB_ao = auto2e_interface_AngFuncTransf(CB, AOBasis, ExternalOrdering)
+    # external AO order -> Auto2e order, Orca phases (Cholesky_Gammcor.f90:73-83)
+    ctx = CholeskyVecsOTF.ctx
+    CA_ao = auto2e_interface_C(CA, AOBasis, ExternalOrdering, ctx)
+    CB_ao = auto2e_interface_C(CB, AOBasis, ExternalOrdering, ctx)
     try:
         out = CholeskyVecsOTF.ctx.mo_transform(CA_ao, a0, a1, CB_ao, b0, b1, MaxBufferDimMB,
                                                lib.MO_ALL_COLUMNS if all_columns else 0)

@@ -18,7 +18,7 @@ EXPORTS = ["gi_create", "gi_destroy", "gi_last_error", "gi_set_basis", "gi_eri_q
            "gi_chol_rkpq_device", "gi_chol_inv_l", "gi_chol_vdiag", "gi_chol_timings",
            "gi_mo_transform", "gi_mo_transform_device", "gi_eri_class_stats", "gi_eri_pair_block",
            "gi_set_option", "gi_chol_build_batch", "gi_coulomb_energy_grad",
-           "gi_chol_residual", "gi_one_electron", "gi_h0"]
+           "gi_chol_residual", "gi_one_electron", "gi_h0", "gi_ao_map", "gi_ao_rows_to_auto2e_device"]
 
 CHOL_DECODE_FIXED = 1
 CHOL_SKIP_VECTORS = 2
@@ -27,6 +27,7 @@ CHOL_RAW_TIES = 8
 CHOL_SPLIT_BF16 = 16
 MO_ALL_COLUMNS = 1
 MO_SPLIT_BF16 = 2
+ORDERING_AUTO2E, ORDERING_MOLPRO, ORDERING_ORCA, ORDERING_DALTON, ORDERING_OPEN_MOLCAS, ORDERING_PYSCF = range(6)
 
 i64 = C.c_int64
 dptr = C.POINTER(C.c_double)
@@ -74,6 +75,8 @@ def load():
     L.gi_mo_transform.argtypes = [C.c_void_p, dptr, i64, i64, i64, dptr, i64, i64, i64, i64, i64, dptr]
     L.gi_mo_transform_device.argtypes = [C.c_void_p, C.c_void_p, i64, i64, i64, C.c_void_p, i64, i64, i64, i64,
                                          i64, C.c_void_p]
+    L.gi_ao_map.argtypes = [C.c_void_p, i64, iptr, dptr]
+    L.gi_ao_rows_to_auto2e_device.argtypes = [C.c_void_p, i64, C.c_void_p, i64, i64, C.c_void_p, i64]
     for name in EXPORTS:
         getattr(L, name).restype = C.c_int if name != "gi_last_error" else C.c_char_p
     _lib = L
@@ -290,6 +293,19 @@ class Context:
                             flags=0):
         _check(self.L.gi_mo_transform_device(self.h, C.c_void_p(CA_ptr), lda, a0, a1, C.c_void_p(CB_ptr), ldb,
                                              b0, b1, max_buffer_mb, flags, C.c_void_p(Rkab_ptr)))
+
+    def ao_map(self, ordering):
+        """(map, phase): map[p] (1-based) = Auto2e position of external AO p,
+        phase in Auto2e order (Orca signs), for the context's basis."""
+        n = self.basis.nao
+        m = np.zeros(n, dtype=np.int64)
+        ph = np.zeros(n)
+        _check(self.L.gi_ao_map(self.h, ordering, _p(m, i64), _p(ph)))
+        return m, ph
+
+    def ao_rows_to_auto2e_device(self, ordering, X_ptr, ld_ext, ncols, Y_ptr, ld_ao):
+        _check(self.L.gi_ao_rows_to_auto2e_device(self.h, ordering, C.c_void_p(X_ptr), ld_ext, ncols,
+                                                  C.c_void_p(Y_ptr), ld_ao))
 
 
 def accuracy_tau(accuracy: int) -> float:

@@ -17,7 +17,8 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import lib
-from .gammcor import GammcorError, auto2e_interface_AngFuncTransf
+from .gammcor import (GammcorError, ORBITAL_ORDERING_ORCA, auto2e_interface_AngFuncTransf,
+                      auto2e_interface_ApplyOrcaPhases_Matrix, auto2e_interface_C)
 
 
 @dataclass
@@ -74,8 +75,11 @@ def ints1e_gammcor_H0_mo(H0_mo, C, AOBasis, System: TSystem, ExternalOrdering: i
     nao = AOBasis.nao_spher if AOBasis.spher_ao else AOBasis.nao_cart
     if C.shape[0] != nao:
         raise GammcorError("Invalid dimension of matrix C")
+    ctx = _context(AOBasis, ctx)
     H0_ao = ints1e_gammcor_H0(None, AOBasis, System, ctx)
-    C_ao = auto2e_interface_AngFuncTransf(C, AOBasis, ExternalOrdering)
+    # MO coefficients in the Auto2e order, Orca phases included
+    # (OneElectronInts_Gammcor.f90:107-113)
+    C_ao = auto2e_interface_C(C, AOBasis, ExternalOrdering, ctx)
     out = C_ao.T @ (H0_ao @ C_ao)
     if H0_mo is not None:
         H0_mo[...] = out
@@ -89,9 +93,14 @@ def ints1e_gammcor_H0_extao(H0_extao, AOBasis, System: TSystem, ExternalOrdering
     nao = AOBasis.nao_spher if AOBasis.spher_ao else AOBasis.nao_cart
     if H0_extao is not None and H0_extao.shape[0] != nao:
         raise GammcorError("Invalid dimension of matrix H0_extao")
+    ctx = _context(AOBasis, ctx)
     H0_ao = ints1e_gammcor_H0(None, AOBasis, System, ctx)
-    out = auto2e_interface_AngFuncTransf(auto2e_interface_AngFuncTransf(H0_ao, AOBasis, ExternalOrdering).T,
-                                         AOBasis, ExternalOrdering).T
+    # Auto2e -> external order on both indices, then the Orca signs applied
+    # at their Auto2e positions to the externally ordered matrix, as the
+    # reference does (OneElectronInts_Gammcor.f90:152-159)
+    out = auto2e_interface_AngFuncTransf(H0_ao, AOBasis, ExternalOrdering, False, True, ctx)
+    if ExternalOrdering == ORBITAL_ORDERING_ORCA:
+        auto2e_interface_ApplyOrcaPhases_Matrix(out, AOBasis, True, ctx)
     if H0_extao is not None:
         H0_extao[...] = out
         return H0_extao

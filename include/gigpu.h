@@ -212,6 +212,33 @@ int gi_mo_transform_device(gi_context *ctx, const double *CA, int64_t lda, int64
                            int64_t a1, const double *CB, int64_t ldb, int64_t b0, int64_t b1,
                            int64_t max_buffer_mb, int64_t flags, double *Rkab);
 
+/* External-program AO orderings (ORBITAL_ORDERING_*, Auto2eInterface.f90:12-17). */
+#define GI_ORDERING_AUTO2E 0
+#define GI_ORDERING_MOLPRO 1
+#define GI_ORDERING_ORCA 2
+#define GI_ORDERING_DALTON 3
+#define GI_ORDERING_OPEN_MOLCAS 4
+#define GI_ORDERING_PYSCF 5
+
+/* auto2e_interface_AOMap (Auto2eInterface.f90:378-514) for the context's
+ * basis: map[p] (1-based) = Auto2e position of the external program's AO p
+ * (Map_extao2ao), nao entries.  phase[q] (Auto2e order) = -1 for the Orca
+ * sign convention of spherical f, g, h functions with m = +-3 and g, h with
+ * m = +-4 (auto2e_interface_ApplyOrcaPhases_Vector, :517-562), +1 otherwise.
+ * Fails like the reference's "Invalid external orbital ordering" (:439,
+ * :488) for GI_ORDERING_AUTO2E and for Cartesian AOs in any ordering
+ * but Molpro's. */
+int gi_ao_map(gi_context *ctx, int64_t ordering, int64_t *map, double *phase);
+/* The rows of a column-major matrix of MO coefficients X_ext (nao x ncols,
+ * ld_ext) from the external AO order to the Auto2e order with the Orca signs
+ * applied, on the device: X_ao(map[p]-1, j) = phase(map[p]-1) X_ext(p, j) --
+ * auto2e_interface_AngFuncTransf + ApplyOrcaPhases_Matrix as
+ * chol_gammcor_Rkab applies them to CA and CB (Cholesky_Gammcor.f90:73-83).
+ * Device pointers, X_ao distinct from X_ext; the result can go straight to
+ * gi_mo_transform_device. */
+int gi_ao_rows_to_auto2e_device(gi_context *ctx, int64_t ordering, const double *X_ext, int64_t ld_ext,
+                                int64_t ncols, double *X_ao, int64_t ld_ao);
+
 #ifdef __cplusplus
 }
 #endif

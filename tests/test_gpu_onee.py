@@ -75,5 +75,7 @@ def test_gammcor_module_mirror(ctx):
     assert np.abs(Hmo - C[:, :12].T @ H @ C[:, :12]).max() <= 1e-12 * np.abs(Hmo).max()
     with pytest.raises(gammcor.GammcorError, match="Invalid dimension of matrix C"):
         oneelectron.ints1e_gammcor_H0_mo(None, C[:-1], basis, system, gammcor.ORBITAL_ORDERING_DALTON, ctx)
-    with pytest.raises(gammcor.GammcorError, match="not supported"):
-        oneelectron.ints1e_gammcor_H0_extao(None, basis, system, gammcor.ORBITAL_ORDERING_ORCA, ctx)
+    # the reference's AO map rejects its own Auto2e ordering
+    # (auto2e_interface_AOMap, Auto2eInterface.f90:439)
+    with pytest.raises(gammcor.GammcorError, match="Invalid external orbital ordering"):
+        oneelectron.ints1e_gammcor_H0_extao(None, basis, system, gammcor.ORBITAL_ORDERING_AUTO2E, ctx)
