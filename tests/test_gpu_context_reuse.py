@@ -67,3 +67,22 @@ def test_split_mo_transform_after_a_larger_one():
         assert np.array_equal(got, ref)
     finally:
         ctx.close()
+
+
+def test_build_without_vectors_drops_earlier_vectors():
+    """Rkpq keeps its device array across builds; a build that stops after
+    Step 2 (GI_CHOL_SKIP_VECTORS) must not leave the earlier vectors
+    readable as if they belonged to it."""
+    b = systems.water_system(1, 6.0, "small", seed=64)
+    ctx = lib.Context(0)
+    try:
+        ctx.set_basis(b)
+        ctx.chol_build(tau=1e-6)
+        assert ctx.chol_rkpq().size > 0
+        ctx.chol_build(tau=1e-6, flags=lib.CHOL_SKIP_VECTORS)
+        with pytest.raises(lib.GigpuError, match="not available"):
+            ctx.chol_rkpq()
+        ctx.chol_build(tau=1e-6)
+        assert ctx.chol_rkpq().size > 0
+    finally:
+        ctx.close()
