@@ -345,7 +345,8 @@ def main():
         "mo_transform_s": t_mo,
         "nvecs": d["nvecs"],
         "eri_quartets": quartets,
-        "phases_s": {k: tm[k] for k in ("vdiag", "step1", "eri_step1", "rtr", "micro", "step2", "vpqrs",
+        "phases_s": {k: tm[k] for k in ("vdiag", "step1", "eri_step1", "rtr", "micro", "candidates", "batch",
+                                        "step2", "vpqrs",
                                         "potrf_trtri", "vectors", "vabrs", "vectors_gemm", "eri_kernel")},
         "rtr_tflops": tm["rtr_flops"] / tm["rtr"] / 1e12 if tm["rtr"] > 0 else None,
         "roofline": {"bound": "mfma",

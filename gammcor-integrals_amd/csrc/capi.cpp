@@ -390,7 +390,7 @@ int gi_chol_residual(gi_context *cp, double *max_residual, double *min_residual)
   })
 }
 
-int gi_chol_timings(gi_context *cp, double t[16]) {
+int gi_chol_timings(gi_context *cp, double t[18]) {
   GI_TRY({
     gi::Context &C = ctx_of(cp);
     auto &T = C.chol.timings;
@@ -411,6 +411,8 @@ int gi_chol_timings(gi_context *cp, double t[16]) {
     t[13] = g("vectors_gemm");
     t[14] = g("eri_launches");
     t[15] = g("rtr_flops");
+    t[16] = g("candidates");
+    t[17] = g("batch");
   })
 }
 

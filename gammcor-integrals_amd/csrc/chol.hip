@@ -755,7 +755,7 @@ void chol_build(Context &C, double tau, double kappa, long long max_block_dim, l
   long long nchol = 0;
   bool converged = false;
   double dmax = 0.0;
-  double t_eri1 = 0, t_rtr = 0, t_micro = 0, rtr_flops = 0;
+  double t_eri1 = 0, t_rtr = 0, t_micro = 0, rtr_flops = 0, t_cand = 0, t_batch = 0;
   {
     double t0 = now();
     long long nsps_cand = nsp, nops_cand = nop;
@@ -780,6 +780,7 @@ void chol_build(Context &C, double tau, double kappa, long long max_block_dim, l
     std::vector<long long> im;
     while (do_macro) {
       S.nmacro++;
+      double tc0 = now();
       // chol2_DefineCandidates: max D per candidate pair, descending sort
       long long n = nsps_cand;
       mv.assign(n + 1, 0.0);
@@ -847,6 +848,8 @@ void chol_build(Context &C, double tau, double kappa, long long max_block_dim, l
         for (long long g = 1; g <= nsps_cand; ++g) loc_comp[cand[g]] = newloc[g];
         ncand = ncomp;
       }
+      t_cand += now() - tc0;
+      double tb0 = now();
       const long long max_nbatch = std::min(nops_cand, max_nbatch_cap);
       double dmin = std::max(0.01 * dmax, tau);
       // chol2_DefineBatch: candidates are already in descending order of their
@@ -874,6 +877,7 @@ void chol_build(Context &C, double tau, double kappa, long long max_block_dim, l
           bfull[colloc[k] - 1 + p] = loc_full[bsp[k]] + p;
         }
       dbd.upload(bd, C.stream);
+      t_batch += now() - tb0;
       // chol2_M: ERIs (candidates | batch)
       dM.alloc((size_t)std::max<long long>(ncand, 1) * std::max<long long>(max_nbatch, 1));
       {
@@ -1003,6 +1007,8 @@ void chol_build(Context &C, double tau, double kappa, long long max_block_dim, l
   S.timings["rtr"] = t_rtr;
   S.timings["rtr_flops"] = rtr_flops;
   S.timings["micro"] = t_micro;
+  S.timings["candidates"] = t_cand;
+  S.timings["batch"] = t_batch;
   S.max_residual = dmax;
   S.nvecs = nchol;
   // step 1 outputs

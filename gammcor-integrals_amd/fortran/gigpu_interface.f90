@@ -149,7 +149,7 @@ module gigpu_c
     integer(c_int) function gi_chol_timings(ctx, t) bind(C, name="gi_chol_timings")
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: ctx
-      real(c_double), intent(out) :: t(16)
+      real(c_double), intent(out) :: t(18)
     end function
     integer(c_int) function gi_chol_rkpq_device(ctx, ptr) bind(C, name="gi_chol_rkpq_device")
       import :: c_int, c_ptr

@@ -266,10 +266,11 @@ class Context:
         return out
 
     def chol_timings(self):
-        t = np.zeros(16)
+        t = np.zeros(18)
         _check(self.L.gi_chol_timings(self.h, _p(t)))
         keys = ["step1", "step2", "vectors", "total", "eri_kernel", "eri_flops", "vdiag", "eri_step1", "rtr",
-                "micro", "vpqrs", "potrf_trtri", "vabrs", "vectors_gemm", "eri_launches", "rtr_flops"]
+                "micro", "vpqrs", "potrf_trtri", "vabrs", "vectors_gemm", "eri_launches", "rtr_flops",
+                "candidates", "batch"]
         return dict(zip(keys, t.tolist()))
 
     def eri_class_stats(self):

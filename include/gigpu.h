@@ -189,8 +189,9 @@ int gi_chol_residual(gi_context *ctx, double *max_residual, double *min_residual
  * operations, [6] Vdiag, [7] step-1 ERIs, [8] M - R^T R, [9] micro
  * iterations, [10] V(P,P) ERIs, [11] POTRF+TRTRI, [12] Vabrs ERIs,
  * [13] Inv_L V^T GEMM, [14] ERI kernel launches (count), [15] M - R^T R
- * FP64 operations. */
-int gi_chol_timings(gi_context *ctx, double t[16]);
+ * FP64 operations, [16] host work of the candidate definition and
+ * compression, [17] host work of the batch definition (both in step 1). */
+int gi_chol_timings(gi_context *ctx, double t[18]);
 
 /* Per-kernel ERI statistics (diagnostic; no reference counterpart), gathered
  * when the environment variable GIGPU_ERI_STATS is set: up to `cap` rows of
