@@ -38,6 +38,41 @@ def test_batch_equals_sequential():
             c.close()
 
 
+def test_batch_of_mixed_bases_equals_sequential():
+    """Uneven entries: uncontracted, contracted and Cartesian bases, large and
+    small, in one batch (the per-context work arrays and class mixes differ
+    from entry to entry)."""
+    mols = [systems.water_system(2, 6.0, "small", seed=81),
+            systems.water_system(1, 6.0, "large", seed=82, contracted=True),
+            systems.water_system(2, 6.0, "small", seed=83, contracted=True, spher_ao=False),
+            systems.water_system(3, 6.0, "small", seed=84, contracted=True)]
+    flags = lib.CHOL_DECODE_FIXED   # Cartesian d shells need the Step-1 decode
+    seq = []
+    ref_ctx = lib.Context(0)
+    try:
+        for b in mols:
+            ref_ctx.set_basis(b)
+            ref_ctx.chol_build(tau=1e-6, flags=flags)
+            seq.append((ref_ctx.chol_meta(), ref_ctx.chol_rkpq()))
+    finally:
+        ref_ctx.close()
+    ctxs = [lib.Context(0) for _ in mols]
+    try:
+        for c, b in zip(ctxs, mols):
+            c.set_basis(b)
+        status = lib.chol_build_batch(ctxs, tau=1e-6, flags=flags, nworkers=4)
+        assert list(status) == [0] * len(mols)
+        for c, (m_ref, r_ref) in zip(ctxs, seq):
+            m = c.chol_meta()
+            assert m["nvecs"] == m_ref["nvecs"]
+            assert np.array_equal(m["pivot_order"], m_ref["pivot_order"])
+            r = c.chol_rkpq()
+            assert np.abs(r - r_ref).max() <= 1e-12 * max(1.0, np.abs(r_ref).max())
+    finally:
+        for c in ctxs:
+            c.close()
+
+
 def test_batch_reports_failing_entry():
     mols = systems.batch_systems(nmol=2, seed=6, nmin=1, nmax=2)
     ctxs = [lib.Context(0) for _ in mols]
