@@ -6,7 +6,7 @@ import numpy as np
 from gigpu.basis import build_basis
 
 
-def tiny_basis(lmax=2, contracted=True):
+def tiny_basis(lmax=2, contracted=True, spher_ao=True):
     """O + H + H with mixed shells (s, p, d; f when lmax = 3) and, when
     contracted, a 3-primitive s and a 2-primitive p shell on O."""
     coords = np.array([[0.0, 0.1, -0.2], [1.4, 1.1, 0.3], [-1.5, 0.9, 0.4]])
@@ -18,7 +18,7 @@ def tiny_basis(lmax=2, contracted=True):
     if lmax >= 3:
         O += [(3, [0.9], [1.0])]
     H = [(0, [0.3], [1.0]), (0, [1.5], [1.0]), (1, [0.75], [1.0])]
-    return build_basis(["O", "H", "H"], coords, {"O": O, "H": H})
+    return build_basis(["O", "H", "H"], coords, {"O": O, "H": H}, spher_ao=spher_ao)
 
 
 def moved(basis, atom, d, h):

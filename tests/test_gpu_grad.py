@@ -59,6 +59,21 @@ def test_gradient_vs_oracle_differences(ctx, contracted, kappa):
     assert np.abs(g - g_fd).max() <= 1e-7, np.abs(g - g_fd).max()
 
 
+@pytest.mark.parametrize("lmax", [2])
+def test_cartesian_energy_and_gradient(ctx, lmax):
+    """Cartesian AOs (no solid-harmonic transform in the pair tables): energy
+    against the oracle, gradient against central differences of the oracle
+    (l <= 2: the gradient kernels take pairs with l0 + l1 <= 5)."""
+    basis = tiny_basis(lmax=lmax, contracted=True, spher_ao=False)
+    D = random_density(basis.nao, seed=7, nocc=6)
+    ctx.set_basis(basis)
+    e, g = ctx.coulomb_energy(D, 0.0)
+    ref = pyoracle.Oracle(basis).coulomb_energy(D, 0.0)
+    assert abs(e - ref) <= 1e-12 * max(1.0, abs(ref))
+    g_fd = fd_gradient(lambda b: pyoracle.Oracle(b).coulomb_energy(D, 0.0), basis)
+    assert np.abs(g - g_fd).max() <= 1e-7, np.abs(g - g_fd).max()
+
+
 def test_gradient_translation_invariance(ctx):
     basis = systems.water_system(3, 6.0, "small", seed=22)
     D = random_density(basis.nao, seed=6, nocc=15)
