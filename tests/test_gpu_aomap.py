@@ -129,3 +129,23 @@ def test_h0_mo_and_extao(gpu_ctx, spher_basis):
         if o == A.ORCA:
             want = want * ph[:, None] * ph[None, :]
         assert np.array_equal(ext, want)
+
+
+def test_device_rows_error_paths(gpu_ctx, spher_basis):
+    """In-place reordering and a leading dimension below NAO are rejected
+    before any kernel runs."""
+    b = spher_basis
+    gpu_ctx.set_basis(b)
+    hip = _Hip()
+    X = hip.upload(np.zeros((b.nao, 2), order="F"))
+    Y = hip.upload(np.zeros((b.nao, 2), order="F"))
+    try:
+        with pytest.raises(lib.GigpuError, match="in-place"):
+            gpu_ctx.ao_rows_to_auto2e_device(A.MOLPRO, X, b.nao, 2, X, b.nao)
+        with pytest.raises(lib.GigpuError, match="Invalid number of atomic orbitals"):
+            gpu_ctx.ao_rows_to_auto2e_device(A.MOLPRO, X, b.nao - 1, 2, Y, b.nao)
+        with pytest.raises(lib.GigpuError, match="Invalid external orbital ordering"):
+            gpu_ctx.ao_rows_to_auto2e_device(A.AUTO2E, X, b.nao, 2, Y, b.nao)
+    finally:
+        hip.free(X)
+        hip.free(Y)
