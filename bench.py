@@ -246,7 +246,8 @@ def main():
                          "size for scaling hardware byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (see DESIGN.md, sharding)",
+        print("bench.py measures one GPU: the library has no multi-GPU path (Step 2's vectors would shard by "
+              "subset; the sequential Step-1 pivot search stays on one GPU)",
               file=sys.stderr)
         sys.exit(2)
     if int(os.environ.get("RANK", "0")) != 0:
