@@ -229,6 +229,8 @@ int gi_chol_build_batch(int64_t n, gi_context *const *ctxs, double tau, double k
     if (n < 0) throw Error{8, "negative batch size"};
     if (n == 0) return 0;
     if (!ctxs || !status) throw Error{8, "null batch arrays"};
+    if (!(tau > 0.0)) throw Error{6, "tau must be positive"};
+    if (max_block_dim <= 0 || vec_block <= 0) throw Error{5, "MaxBlockDim and CholVecsBlock must be positive"};
     std::atomic<int64_t> next{0};
     std::vector<std::string> msgs(n);
     auto worker = [&]() {
@@ -280,10 +282,12 @@ int gi_h0(gi_context *cp, const double *atom_charges, int64_t npc, const double 
     gi::Context &C = ctx_of(cp);
     if (!H0) throw Error{8, "null H0"};
     const long long n = C.nao;
-    std::vector<double> Tm((size_t)n * n);
+    if (ldh < n) throw Error{8, "leading dimension smaller than the number of AOs"};
+    // T is written with the caller's leading dimension, like H0
+    std::vector<double> Tm((size_t)ldh * (n - 1) + n);
     gi::one_electron(C, atom_charges, npc, pc_coords, pc_charges, nullptr, Tm.data(), H0, ldh);
     for (long long j = 0; j < n; ++j)
-      for (long long i = 0; i < n; ++i) H0[i + ldh * j] += Tm[i + n * j];
+      for (long long i = 0; i < n; ++i) H0[i + ldh * j] += Tm[i + ldh * j];
   })
 }
 
@@ -312,6 +316,7 @@ int gi_chol_dims(gi_context *cp, int64_t dims[8]) {
   GI_TRY({
     gi::Context &C = ctx_of(cp);
     const gi::CholState &S = C.chol;
+    if (!S.valid) throw Error{50, "no Cholesky decomposition is available (run gi_chol_build first)"};
     dims[0] = S.nvecs;
     dims[1] = S.norb_pairs;
     dims[2] = S.nshell_pairs;
@@ -334,6 +339,7 @@ int gi_chol_meta(gi_context *cp, int64_t *shell_pairs, int64_t *shell_pair_loc, 
   GI_TRY({
     gi::Context &C = ctx_of(cp);
     const gi::CholState &S = C.chol;
+    if (!S.valid) throw Error{50, "no Cholesky decomposition is available (run gi_chol_build first)"};
     copy_vec(shell_pairs, S.shell_pairs);
     copy_vec(shell_pair_loc, S.shell_pair_loc);
     copy_vec(shell_pair_dim, S.shell_pair_dim);
@@ -369,7 +375,7 @@ int gi_chol_rkpq_device(gi_context *cp, double **ptr) {
 int gi_chol_inv_l(gi_context *cp, double *dst) {
   GI_TRY({
     gi::Context &C = ctx_of(cp);
-    if (!C.chol.inv_l.p) throw Error{50, "Inv_L is not available"};
+    if (!C.chol.valid || !C.chol.inv_l.p) throw Error{50, "Inv_L is not available"};
     GI_HIP(hipMemcpy(dst, C.chol.inv_l.p, sizeof(double) * C.chol.nvecs * C.chol.nvecs, hipMemcpyDeviceToHost));
   })
 }
@@ -377,7 +383,7 @@ int gi_chol_inv_l(gi_context *cp, double *dst) {
 int gi_chol_vdiag(gi_context *cp, double *dst) {
   GI_TRY({
     gi::Context &C = ctx_of(cp);
-    if (C.chol.vdiag.empty()) throw Error{50, "Vdiag is not available"};
+    if (!C.chol.valid || C.chol.vdiag.empty()) throw Error{50, "Vdiag is not available"};
     std::memcpy(dst, C.chol.vdiag.data(), sizeof(double) * C.chol.vdiag.size());
   })
 }

@@ -11,8 +11,9 @@
 //
 // Everything O(N^3..N^4) runs on the device: ERIs (class kernels), the
 // candidate compaction, M - L^T L (rocBLAS DGEMM), the micro-iteration
-// column update + diagonal update + batch argmax (own kernels), POTRF/TRTRI
-// (rocSOLVER) and Inv_L V^T (strided-batched DGEMM).  The host keeps the
+// column update + diagonal update + batch argmax (own kernels), the Cholesky
+// factor of V(P,P) and its inverse (own recursive factorization: LDS leaf
+// kernel + rocBLAS DGEMMs) and Inv_L V^T (triangular-blocked DGEMMs).  The host keeps the
 // bookkeeping whose order fixes the pivot sequence: the shell-pair sorts
 // (Singleton quicksort, identical permutation for tied keys), the BASE
 // screening, the batch selection and the strict-> first-maximum rule.
@@ -69,10 +70,14 @@ __global__ void k_segmax(const double *__restrict__ D, const long long *__restri
 // order differences reach 1e-11 relative on small diagonals, 1e-13 absolute
 // on residuals) and well below tau.  BASE sort keys: mantissa rounded to
 // KEY_REL_BITS bits.  Residual diagonals: absolute quantum
-// delta = 2^(floor(log2 MaxDrs) - KEY_ABS_BITS), a power of two so x/delta is
-// exact.
+// delta = min(2^(floor(log2 MaxDrs) - KEY_ABS_BITS), 2^(floor(log2 tau) -
+// KEY_TAU_BITS)), a power of two so x/delta is exact.  Only orderings use the
+// quantized keys; the thresholds (tau for candidates and convergence, Dmin
+// for the batch and the micro iterations) compare raw values, as the
+// reference does.
 constexpr int KEY_REL_BITS = 26;
 constexpr int KEY_ABS_BITS = 32;
+constexpr int KEY_TAU_BITS = 8;
 static double qrel(double x) {
   if (x == 0.0 || !std::isfinite(x)) return x;
   int e;
@@ -672,7 +677,9 @@ void chol_build(Context &C, double tau, double kappa, long long max_block_dim, l
   const bool raw_ties = (flags & 8) != 0;
   int maxdrs_exp = 0;
   std::frexp(maxdrs, &maxdrs_exp);
-  const double inv_delta = std::ldexp(1.0, KEY_ABS_BITS + 1 - maxdrs_exp);   // 1 / delta
+  int tau_exp = 0;
+  std::frexp(tau, &tau_exp);
+  const double inv_delta = std::ldexp(1.0, std::max(KEY_ABS_BITS + 1 - maxdrs_exp, KEY_TAU_BITS + 1 - tau_exp));
   if (!raw_ties)
     for (long long k = 1; k <= max_sp; ++k) vd_max[k] = qrel(vd_max[k]);
   dsort0(vd_max.data() + 1, idxmap.data() + 1, max_sp, 2);
@@ -778,6 +785,7 @@ void chol_build(Context &C, double tau, double kappa, long long max_block_dim, l
     DBuf<int> &ddim = cw.dim;
     std::vector<double> mv;
     std::vector<long long> im;
+    std::vector<double> spmax(nsp + 1, 0.0);   // raw residual maximum per BASE shell pair
     while (do_macro) {
       S.nmacro++;
       double tc0 = now();
@@ -798,18 +806,22 @@ void chol_build(Context &C, double tau, double kappa, long long max_block_dim, l
         GI_HIP(hipStreamSynchronize(C.stream));
         for (long long k = 1; k <= n; ++k) {
           im[k] = cand[k];
+          spmax[cand[k]] = mv[k];
           if (!raw_ties) mv[k] = qabs(mv[k], inv_delta) / inv_delta;
         }
       }
+      // sorted by the (quantized) key; the threshold compares the raw maxima,
+      // which within one quantum need not be monotone, so every pair is tested
+      // (with raw keys this is the reference's loop up to its first miss)
       dsort0(mv.data() + 1, im.data() + 1, n, -2);
       nsps_cand = 0;
       nops_cand = 0;
       for (long long k = 1; k <= n; ++k) {
-        if (mv[k] > tau) {
+        if (spmax[im[k]] > tau) {
           ++nsps_cand;
-          cand[k] = im[k];
+          cand[nsps_cand] = im[k];
           nops_cand += spdim[im[k]];
-        } else break;
+        } else if (raw_ties) break;
       }
       // chol2_Compress: move D and the stored vectors to the new candidate order
       long long max_nbatch_cap = std::max<long long>((long long)maxnang * maxnang, 1000);
@@ -856,13 +868,16 @@ void chol_build(Context &C, double tau, double kappa, long long max_block_dim, l
       // maxima; the reference sorts them again (ties may be permuted)
       std::vector<double> ds(nsps_cand + 1);
       std::vector<long long> di(nsps_cand + 1);
-      for (long long k = 1; k <= nsps_cand; ++k) { ds[k] = mv[k]; di[k] = cand[k]; }
+      for (long long k = 1; k <= nsps_cand; ++k) {
+        di[k] = cand[k];
+        ds[k] = raw_ties ? spmax[cand[k]] : qabs(spmax[cand[k]], inv_delta) / inv_delta;
+      }
       dsort0(ds.data() + 1, di.data() + 1, nsps_cand, -2);
       long long nsps_batch = 0, nops_batch = 0;
       std::vector<long long> bsp(1), colloc(1);
       for (long long k = 1; k <= nsps_cand; ++k) {
         long long ab = di[k];
-        if (ds[k] > dmin && nops_batch + spdim[ab] <= max_nbatch) {
+        if (spmax[ab] > dmin && nops_batch + spdim[ab] <= max_nbatch) {
           bsp.push_back(ab);
           colloc.push_back(nops_batch + 1);
           nsps_batch = k;
@@ -996,8 +1011,9 @@ void chol_build(Context &C, double tau, double kappa, long long max_block_dim, l
         nchol += J;
       }
       t_micro += now() - tm;
+      // maximum of the raw residual diagonal over the rows still held (with no
+      // candidate left these are the previous candidates, all <= tau)
       dmax = (ncand > 0) ? device_max(C, dD.p, ncand, dtmp) : 0.0;
-      if (nsps_cand == 0) dmax = 0.0;
       if (dmax <= tau) { converged = true; do_macro = false; }
       if (nchol == max_nchol || nops_batch == 0) do_macro = false;
     }

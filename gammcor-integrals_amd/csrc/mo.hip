@@ -112,7 +112,7 @@ void mo_transform(Context &C, const double *dCA, long long lda, long long a0, lo
   long long maxnk = std::min(NA, maxbuf / std::max<long long>(nvec * nao, 1));
   if (!(maxnk > 0)) throw Error{53, "Buffer size too small for Rpq->Rab"};
   std::vector<char> done(NA, 0);
-  long long npasses = NA / maxnk + (NA % maxnk > 0);
+  const long long npasses = NA / maxnk + (NA % maxnk > 0);
   for (long long k = 1; k <= npasses; ++k) {
     long long a0k = (k - 1) * maxnk + a0, a1k = std::min(k * maxnk, a1);
     for (long long a = a0k; a <= a1k; ++a) done[a - a0] = 1;
@@ -148,22 +148,46 @@ void mo_transform(Context &C, const double *dCA, long long lda, long long a0, lo
     S.mo_adj_col.upload(cols, C.stream);
     S.mo_adj_p.upload(ps, C.stream);
   }
-  const long long ldct = (NA + 63) / 64 * 64;
+  // passes over a-column ranges, W(k, a, q) of at most maxnk columns
+  // (the reference's NCholesky*MaxNk*NAO buffer, :186); GI_MO_ALL_COLUMNS
+  // takes contiguous ranges covering a0..a1, otherwise the reference's pass
+  // bounds (columns they skip stay zero)
+  std::vector<std::pair<long long, long long>> passes;
+  for (long long k = 1; k <= npasses; ++k) {
+    long long a0k, a1k;
+    if (flags & 1) {
+      a0k = a0 + (k - 1) * maxnk;
+      a1k = std::min(a0k + maxnk - 1, a1);
+    } else {
+      a0k = (k - 1) * maxnk + a0;
+      a1k = std::min(k * maxnk, a1);
+    }
+    if (a1k >= a0k) passes.push_back({a0k, a1k});
+  }
+  long long nk_max = 0;
+  for (auto &pr : passes) nk_max = std::max(nk_max, pr.second - pr.first + 1);
+  const long long ldct = (std::max<long long>(nk_max, 1) + 63) / 64 * 64;
   DBuf<double> dCT, dW;
   dCT.alloc((size_t)ldct * nao);
-  dW.alloc((size_t)nvec * NA * nao);
-  hipLaunchKernelGGL(k_mo_ct, dim3((unsigned)(ldct / 64), (unsigned)nao), dim3(64), 0, C.stream, dCT.p, ldct, dCA,
-                     lda, a0, NA, nao);
-  hipLaunchKernelGGL(k_mo_half, dim3((unsigned)nao, (unsigned)((nvec + 63) / 64), (unsigned)(ldct / 64)), dim3(64), 0,
-                     C.stream, S.rkpq.p, nvec, S.mo_adj_off.p, S.mo_adj_col.p, S.mo_adj_p.p, dCT.p, ldct, NA, dW.p);
-  GI_HIP(hipGetLastError());
+  dW.alloc((size_t)nvec * std::max<long long>(nk_max, 1) * nao);
   const double one = 1.0, zero = 0.0;
-  if (flags & 2)
-    split_bf16_gemm(C, rocblas_operation_none, rocblas_operation_none, nvec * NA, NB, nao, 1.0, dW.p, nvec * NA,
-                    dCB + (b0 - 1) * ldb, ldb, 0.0, dRkab, nvec * NA);
-  else
-    RB_CHECK(rocblas_dgemm(C.blas, rocblas_operation_none, rocblas_operation_none, nvec * NA, NB, nao, &one, dW.p,
-                           nvec * NA, dCB + (b0 - 1) * ldb, ldb, &zero, dRkab, nvec * NA));
+  for (auto &pr : passes) {
+    const long long a0k = pr.first, Nk = pr.second - pr.first + 1;
+    hipLaunchKernelGGL(k_mo_ct, dim3((unsigned)(ldct / 64), (unsigned)nao), dim3(64), 0, C.stream, dCT.p, ldct,
+                       dCA, lda, a0k, Nk, nao);
+    hipLaunchKernelGGL(k_mo_half, dim3((unsigned)nao, (unsigned)((nvec + 63) / 64), (unsigned)((Nk + 63) / 64)),
+                       dim3(64), 0, C.stream, S.rkpq.p, nvec, S.mo_adj_off.p, S.mo_adj_col.p, S.mo_adj_p.p, dCT.p,
+                       ldct, Nk, dW.p);
+    GI_HIP(hipGetLastError());
+    // Rkab(k, a, b), a in this pass: rows nvec (a0k - a0) .. of each b column
+    double *dst = dRkab + nvec * (a0k - a0);
+    if (flags & 2)
+      split_bf16_gemm(C, rocblas_operation_none, rocblas_operation_none, nvec * Nk, NB, nao, 1.0, dW.p, nvec * Nk,
+                      dCB + (b0 - 1) * ldb, ldb, 0.0, dst, nvec * NA);
+    else
+      RB_CHECK(rocblas_dgemm(C.blas, rocblas_operation_none, rocblas_operation_none, nvec * Nk, NB, nao, &one, dW.p,
+                             nvec * Nk, dCB + (b0 - 1) * ldb, ldb, &zero, dst, nvec * NA));
+  }
   if (!(flags & 1)) {
     std::vector<long long> skip;
     for (long long a = 0; a < NA; ++a)

@@ -217,25 +217,29 @@ class Context:
         qq = np.ascontiguousarray(pc[:, 3])
         return q, len(pc), xyz, qq
 
-    def one_electron(self, charges, point_charges=None):
+    def one_electron(self, charges, point_charges=None, ld=None):
         """(S, T, V) in the AO basis: gi_one_electron.  charges: one per atom;
-        point_charges: rows (x, y, z, q), bohr."""
+        point_charges: rows (x, y, z, q), bohr; ld: leading dimension of the
+        output arrays (>= nao; the returned views are nao x nao)."""
         n = self.basis.nao
+        ld = n if ld is None else ld
         q, npc, xyz, qq = self._charges(self.basis.natoms, charges, point_charges)
-        S, T, V = (np.zeros((n, n), order="F") for _ in range(3))
+        S, T, V = (np.zeros((ld, n), order="F") for _ in range(3))
         _check(self.L.gi_one_electron(self.h, None if q is None else q.ctypes.data, npc,
                                       xyz.ctypes.data if npc else None, qq.ctypes.data if npc else None,
-                                      S.ctypes.data, T.ctypes.data, V.ctypes.data, n))
-        return S, T, V
+                                      S.ctypes.data, T.ctypes.data, V.ctypes.data, ld))
+        return S[:n], T[:n], V[:n]
 
-    def h0(self, charges, point_charges=None):
-        """H0 = T + V_ne in the AO basis: gi_h0 (ints1e_gammcor_H0)."""
+    def h0(self, charges, point_charges=None, ld=None):
+        """H0 = T + V_ne in the AO basis: gi_h0 (ints1e_gammcor_H0); ld as in
+        one_electron."""
         n = self.basis.nao
+        ld = n if ld is None else ld
         q, npc, xyz, qq = self._charges(self.basis.natoms, charges, point_charges)
-        H = np.zeros((n, n), order="F")
+        H = np.zeros((ld, n), order="F")
         _check(self.L.gi_h0(self.h, None if q is None else q.ctypes.data, npc, xyz.ctypes.data if npc else None,
-                            qq.ctypes.data if npc else None, _p(H), n))
-        return H
+                            qq.ctypes.data if npc else None, _p(H), ld))
+        return H[:n]
 
     def chol_residual(self):
         """(max, min) over BASE orbital pairs of (pq|pq) - sum_k Rkpq(k, pq)^2."""

@@ -23,13 +23,21 @@
  *   chol2_Vabrs                       TwoStepCholesky_Step2.f90:35-73,242-419
  *   chol2_FullDimVectors_Batch        TwoStepCholesky.f90:378-416
  *
- * Pinned by tests/test_oracle_chol.py: a brute-force textbook pivoted
- * Cholesky on a dense V of a tiny basis reproducing the same heuristics
- * (sigma=0.01 batches, strict > first maximum) must give the same pivot
- * sequence; residual max diag <= tau; R^T R reconstructs V within tau on the
- * diagonal and (for the fixed decode) everywhere; dense-NumPy Step 2 from the
- * pivot list gives the same Inv_L and Rkpq; the pq decode matches the
- * traversal drawn in TwoStepCholesky_Step2.f90:432-436.
+ * Pinned by:
+ *   tests/test_oracle_pivots.py   the pivot sequence, BASE shell pairs and
+ *                                 vector count equal a dense NumPy restatement
+ *                                 of Step 1 written from the Fortran
+ *                                 (tests/dense_pivots.py), on uncontracted,
+ *                                 contracted, prescreened and 3240-pair
+ *                                 systems, quantized and raw keys;
+ *   tests/test_oracle_chol.py     R^T R reproduces the factorized columns of
+ *                                 V; V - R^T R within tau (fixed decode);
+ *                                 Inv_L is the inverse Cholesky factor of
+ *                                 V(P,P); subsets do not change the pivots;
+ *                                 tau = 1e-10 converges below tau;
+ *   tests/test_oracle_step2_mo.py the pq decode against the N=3 traversal
+ *                                 drawn at TwoStepCholesky_Step2.f90:432-436;
+ *                                 the MO transform against C^T L^k C.
  */
 #include <math.h>
 #include <stdio.h>
@@ -55,13 +63,17 @@ static double wall(void) { return omp_get_wtime(); }
  *     (26) bits;
  *   - residual diagonals (candidate/batch sorts, batch argmax): rounded to
  *     an absolute quantum delta = 2^(floor(log2 MaxDrs) - KEY_ABS_BITS),
- *     KEY_ABS_BITS = 32.
+ *     KEY_ABS_BITS = 32, or 2^(floor(log2 tau) - KEY_TAU_BITS), KEY_TAU_BITS
+ *     = 8, whichever is smaller, so that the quantum stays far below tau.
+ * Only the orderings use the quantized keys: the thresholds (tau for the
+ * candidates, Dmin for the batch and the micro iterations, tau for
+ * convergence) compare the raw values, as the reference does.
  * The quanta sit above the disagreement of two correct ERI evaluations
  * (different Boys-function evaluation and summation order reach 1e-11
  * relative on small diagonals) and far below tau.
  * Quantized ties fall back to the sort's tie order and the first-in-batch
  * rule.  OR_CHOL_RAW_TIES restores raw comparisons. */
-enum { KEY_REL_BITS = 26, KEY_ABS_BITS = 32 };
+enum { KEY_REL_BITS = 26, KEY_ABS_BITS = 32, KEY_TAU_BITS = 8 };
 
 static double qrel(double x)
 {
@@ -204,7 +216,11 @@ void or_chol_build(const or_basis *b, double tau, double kappa, int64_t max_bloc
     const int raw_ties = (flags & OR_CHOL_RAW_TIES) != 0;
     int maxdrs_exp;
     frexp(maxdrs, &maxdrs_exp);
-    const double inv_delta = ldexp(1.0, KEY_ABS_BITS + 1 - maxdrs_exp);   /* 1 / delta */
+    int tau_exp;
+    frexp(tau, &tau_exp);
+    /* 1 / delta, delta = min(2^(floor(log2 MaxDrs) - KEY_ABS_BITS), 2^(floor(log2 tau) - KEY_TAU_BITS)) */
+    const double inv_delta = ldexp(1.0, (KEY_ABS_BITS + 1 - maxdrs_exp) > (KEY_TAU_BITS + 1 - tau_exp)
+                                            ? (KEY_ABS_BITS + 1 - maxdrs_exp) : (KEY_TAU_BITS + 1 - tau_exp));
     if (!raw_ties)
         for (int64_t k = 1; k <= max_sp; ++k) vd_max[k] = qrel(vd_max[k]);
     or_dsort0(vd_max + 1, idxmap + 1, max_sp, 2);
@@ -290,6 +306,7 @@ void or_chol_build(const or_basis *b, double tau, double kappa, int64_t max_bloc
         int64_t *spmap = (int64_t *)calloc(3 * (nsp + 1) + 3, sizeof(int64_t)); /* spmap[k + (nsp+1)*(mode-1)] */
 #define SPMAP(k, mode) spmap[(k) + (nsp + 1) * ((mode) - 1)]
         int64_t *colloc = (int64_t *)calloc(nsp + 1, sizeof(int64_t));
+        double *spmax = (double *)calloc(nsp + 1, sizeof(double));   /* raw residual maximum per shell pair */
         int64_t max_nblocks = max_nchol / block_dim + (max_nchol % block_dim > 0);
         double **Vb = (double **)calloc(max_nblocks + 1, sizeof(double *));
         for (int64_t k = 1; k <= nsp; ++k) { SPMAP(k, CAND) = k; sploc[5 * k + COMPR] = sploc[5 * k + FULL]; }
@@ -298,6 +315,7 @@ void or_chol_build(const or_basis *b, double tau, double kappa, int64_t max_bloc
         int64_t cur_block = 0, block_boundary = 0;
         int do_macro = max_nchol > 0;
         double *Dc = D;  /* current compressed D, 1-based */
+        int64_t nD = nop; /* its length */
         while (do_macro) {
             res->nmacro++;
             double tc = wall();
@@ -310,18 +328,23 @@ void or_chol_build(const or_basis *b, double tau, double kappa, int64_t max_bloc
                     int64_t ab = SPMAP(k, CAND), loc = sploc[5 * ab + COMPR];
                     double mx = -INFINITY;
                     for (int64_t i = 0; i < spdim[ab]; ++i) if (Dc[loc + i] > mx) mx = Dc[loc + i];
+                    spmax[ab] = mx;
                     mv[k] = raw_ties ? mx : qabs(mx, inv_delta) / inv_delta;
                     im[k] = ab;
                 }
                 or_dsort0(mv + 1, im + 1, n, -2);
                 nsps[CAND] = 0;
                 nops[CAND] = 0;
+                /* sorted by the (quantized) key, thresholded on the raw maximum:
+                 * within one quantum the raw values need not be monotone, so
+                 * every pair is tested (raw keys: the reference's loop up to
+                 * its first miss) */
                 for (int64_t k = 1; k <= n; ++k) {
-                    if (mv[k] > tau) {
+                    if (spmax[im[k]] > tau) {
                         nsps[CAND]++;
-                        SPMAP(k, CAND) = im[k];
+                        SPMAP(nsps[CAND], CAND) = im[k];
                         nops[CAND] += spdim[im[k]];
-                    } else break;
+                    } else if (raw_ties) break;
                 }
                 free(mv); free(im);
             }
@@ -345,6 +368,7 @@ void or_chol_build(const or_basis *b, double tau, double kappa, int64_t max_bloc
                 if (Dc != D) free(Dc); else free(D);
                 D = NULL;
                 Dc = Dn;
+                nD = ncomp;
                 if (nchol > 0) {
                     int64_t nbl = nchol / block_dim + (nchol % block_dim > 0);
                     for (int64_t bl = 1; bl <= nbl; ++bl) {
@@ -378,6 +402,7 @@ void or_chol_build(const or_basis *b, double tau, double kappa, int64_t max_bloc
                     int64_t ab = SPMAP(k, CAND), loc = sploc[5 * ab + COMPR];
                     double mx = -INFINITY;
                     for (int64_t i = 0; i < spdim[ab]; ++i) if (Dc[loc + i] > mx) mx = Dc[loc + i];
+                    spmax[ab] = mx;
                     ds[k] = raw_ties ? mx : qabs(mx, inv_delta) / inv_delta;
                     di[k] = ab;
                 }
@@ -386,7 +411,7 @@ void or_chol_build(const or_basis *b, double tau, double kappa, int64_t max_bloc
                 nops[BATCH] = 0;
                 for (int64_t k = 1; k <= n; ++k) {
                     int64_t ab = di[k];
-                    if (ds[k] > dmin && nops[BATCH] + spdim[ab] <= max_nbatch) {
+                    if (spmax[ab] > dmin && nops[BATCH] + spdim[ab] <= max_nbatch) {
                         int64_t loc = nops[BATCH] + 1;
                         nsps[BATCH] = k;
                         nops[BATCH] += spdim[ab];
@@ -494,9 +519,11 @@ void or_chol_build(const or_basis *b, double tau, double kappa, int64_t max_bloc
                 if (j == nops[BATCH] || nchol == max_nchol) do_micro = 0;
             }
             res->t_rk += wall() - trk;
+            /* raw residual maximum over the rows of D still held (no candidate
+             * left: the previous candidates, all <= tau) */
             dmax = -INFINITY;
-            for (int64_t i = 1; i <= ncand; ++i) if (Dc[i] > dmax) dmax = Dc[i];
-            if (ncand == 0) dmax = 0.0;
+            for (int64_t i = 1; i <= nD; ++i) if (Dc[i] > dmax) dmax = Dc[i];
+            if (nD == 0) dmax = 0.0;
             if (dmax <= tau) { converged = 1; do_macro = 0; }
             if (nchol == max_nchol || nops[BATCH] == 0) do_macro = 0;
             free(M);
@@ -506,6 +533,7 @@ void or_chol_build(const or_basis *b, double tau, double kappa, int64_t max_bloc
         for (int64_t bl = 1; bl <= max_nblocks; ++bl) free(Vb[bl]);
         free(Vb);
         free(colloc);
+        free(spmax);
         free(spmap);
 #undef SPMAP
         if (Dc != D) free(Dc);

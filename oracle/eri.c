@@ -424,3 +424,42 @@ void or_eri_quartet_single_boys(const or_basis *b, int64_t sa, int64_t sb, int64
 {
     eri_any(b, sa, sb, sc, sd, kappa, 1, V);
 }
+
+/* Dense (pq|rs) over a list of shell pairs (0-based (A, B) with A >= B), the
+ * orbital pairs of each in the reference's storage order (b outer, a inner,
+ * a >= b when A == B; chol2_write_Vdiag, TwoStepCholesky_Step1.f90:175-199).
+ * V is n x n column-major, n = sum of the pair dimensions.  Test helper for
+ * dense restatements of the decomposition. */
+void or_eri_pair_matrix(const or_basis *b, int64_t npairs, const int64_t *pairs, double kappa, double *V, int64_t n)
+{
+    int64_t *off = (int64_t *)malloc(sizeof(int64_t) * (npairs + 1));
+    off[0] = 0;
+    for (int64_t i = 0; i < npairs; ++i) {
+        int64_t na = or_nang(b, pairs[2 * i]), nb = or_nang(b, pairs[2 * i + 1]);
+        off[i + 1] = off[i] + (pairs[2 * i] == pairs[2 * i + 1] ? na * (na + 1) / 2 : na * nb);
+    }
+    #pragma omp parallel for schedule(dynamic)
+    for (int64_t j = 0; j < npairs; ++j) {
+        const int64_t sc = pairs[2 * j], sd = pairs[2 * j + 1];
+        const int64_t nc = or_nang(b, sc), nd = or_nang(b, sd);
+        for (int64_t i = j; i < npairs; ++i) {
+            const int64_t sa = pairs[2 * i], sb = pairs[2 * i + 1];
+            const int64_t na = or_nang(b, sa), nb = or_nang(b, sb);
+            double *blk = (double *)malloc(sizeof(double) * na * nb * nc * nd);
+            or_eri_quartet(b, sa, sb, sc, sd, kappa, blk);
+            int64_t q = off[j];
+            for (int64_t d = 0; d < nd; ++d)
+                for (int64_t c = (sc == sd ? d : 0); c < nc; ++c, ++q) {
+                    int64_t p = off[i];
+                    for (int64_t bb = 0; bb < nb; ++bb)
+                        for (int64_t a = (sa == sb ? bb : 0); a < na; ++a, ++p) {
+                            const double v = blk[a + na * (bb + nb * (c + nc * d))];
+                            V[p + n * q] = v;
+                            V[q + n * p] = v;
+                        }
+                }
+            free(blk);
+        }
+    }
+    free(off);
+}

@@ -63,6 +63,11 @@ void or_eri_quartet(const or_basis *b, int64_t sa, int64_t sb, int64_t sc, int64
 void or_eri_quartet_single_boys(const or_basis *b, int64_t sa, int64_t sb, int64_t sc,
                                 int64_t sd, double kappa, double *V);
 
+/* Dense (pq|rs) over shell pairs (0-based (A,B), A >= B), reference storage
+ * order inside each pair; V n x n column-major. */
+void or_eri_pair_matrix(const or_basis *b, int64_t npairs, const int64_t *pairs, double kappa, double *V,
+                        int64_t n);
+
 /* ---- sorting (common/sort.f90:19-330) ---- */
 void or_dsort0(double *dx, int64_t *dy, int64_t n, int64_t kflag);
 

@@ -77,6 +77,7 @@ def lib():
                                       C.c_void_p, i64]
         L.or_coulomb_energy.argtypes = [C.POINTER(OrBasis), dptr, i64, C.c_double]
         L.or_coulomb_energy.restype = C.c_double
+        L.or_eri_pair_matrix.argtypes = [C.POINTER(OrBasis), i64, iptr, C.c_double, dptr, i64]
         L.or_dsort0.argtypes = [dptr, iptr, i64, i64]
         L.or_chol_build.argtypes = [C.POINTER(OrBasis), C.c_double, C.c_double, i64, i64, i64,
                                     C.POINTER(OrCholResult)]
@@ -121,6 +122,20 @@ class Oracle:
         f = self.L.or_eri_quartet_single_boys if single_boys else self.L.or_eri_quartet
         f(C.byref(self.ob), sa, sb, sc, sd, kappa, _p(V))
         return V.reshape(n[3], n[2], n[1], n[0]).transpose(3, 2, 1, 0)
+
+    def pair_matrix(self, shell_pairs, kappa=0.0):
+        """Dense (pq|rs) over the orbital pairs of shell_pairs (1-based
+        (A, B), A >= B), each pair's orbital pairs in the reference's storage
+        order (b outer, a inner, a >= b for A == B)."""
+        sp = np.ascontiguousarray(np.asarray(shell_pairs, dtype=np.int64).reshape(-1, 2) - 1)
+        dims = []
+        for a, b in sp:
+            na, nb = self.basis.shell_nang(int(a)), self.basis.shell_nang(int(b))
+            dims.append(na * (na + 1) // 2 if a == b else na * nb)
+        n = int(sum(dims))
+        V = np.zeros((n, n), order="F")
+        self.L.or_eri_pair_matrix(C.byref(self.ob), len(sp), _p(sp, i64), float(kappa), _p(V), n)
+        return V
 
     def one_electron(self, charges, point_charges=None):
         """(S, T, V) in the AO basis; charges per atom, point_charges an

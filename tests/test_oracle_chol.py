@@ -110,3 +110,20 @@ def test_cartesian_same_shell_pairs_are_dependent(oracle_mod):
             assert np.allclose(G[xx, yy], G[xy, xy] / 3.0, rtol=1e-13, atol=1e-15)
     assert o.chol(tau=1e-6).status == 40
     assert o.chol(tau=1e-6, flags=1).status == 0
+
+
+@pytest.mark.parametrize("n,basis", [(2, "small"), (1, "large")])
+def test_debug_accuracy_converges_below_tau(oracle_mod, n, basis):
+    """tau = 1e-10 (CHOL_ACCURACY_DEBUG) is below the residual quantum the
+    orderings would use at 2^-32 of MaxDrs; the thresholds compare raw values,
+    so the build converges with max residual <= tau, and the quantized and raw
+    orderings keep the same number of vectors."""
+    b = systems.water_system(n, 6.0, basis, seed=1)
+    o = oracle_mod.Oracle(b)
+    q = o.chol(tau=1e-10, flags=1 | 2)
+    r = o.chol(tau=1e-10, flags=1 | 2 | 4)
+    for x in (q, r):
+        assert x.status == 0, x.message
+        assert 0.0 < x.max_residual <= 1e-10
+    assert q.nvecs == r.nvecs
+    assert q.nvecs < 15 * b.nao
