@@ -9,6 +9,7 @@
 // (x, y, z) for p; Cartesian functions run lx = l..0, ly = l - lx..0.  The
 // shells themselves are in the same order in both programs, except for Open
 // Molcas, which orders the AOs of each atom by l, then m, then shell.
+#include <mutex>
 #include "gi_context.h"
 
 namespace gi {
@@ -143,6 +144,64 @@ void ao_rows_to_auto2e(Context &C, int ordering, const double *X, long long ldx,
   const unsigned grid = (unsigned)std::min<long long>((total + 255) / 256, 65536);
   hipLaunchKernelGGL(k_ao_rows, dim3(grid), dim3(256), 0, C.stream, X, ldx, (long long)C.nao, ncols, dmap.p, dph.p,
                      Y, ldy);
+  GI_HIP(hipGetLastError());
+  GI_HIP(hipStreamSynchronize(C.stream));
+}
+
+// ---- allocation-free variant (gi_ao_rows_to_auto2e_device_noalloc)
+// The map and the signs are staged in a file-scope device array instead of
+// stream-ordered allocations: NOALLOC_MAX_NAO map entries (int64) followed by
+// NOALLOC_MAX_NAO signs (double), 1 MiB, held on every device the library
+// runs on until the process ends.  Host code never passes `scratch` itself
+// to a kernel or copy (in host code the name is a stand-in, not a device
+// address): the device address comes from hipGetSymbolAddress, taken once per
+// device with that device current (noalloc_setup, called by gi_create), and
+// only that address is used.  Every call on one device shares the array, so
+// overlapping calls on one device race; each call writes the whole map and
+// sign range it reads, so nothing has to start at zero.
+constexpr long long NOALLOC_MAX_NAO = 1LL << 16;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_MAX_NAO * (sizeof(long long) + sizeof(double))];
+
+namespace {
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+}  // namespace
+
+// with `device` current: fetch the device address of `scratch` once per device
+void noalloc_setup(int device) {
+  if (device < 0 || device >= NOALLOC_MAX_DEVICES) throw Error{1, "device index beyond the library's table"};
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[device]) return;
+  void *p = nullptr;
+  GI_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  if (!p) throw Error{2, "hipGetSymbolAddress returned no address for the scratch array"};
+  g_scratch_ptr[device] = p;
+}
+
+void ao_rows_to_auto2e_noalloc(Context &C, int ordering, const double *X, long long ldx, long long ncols, double *Y,
+                               long long ldy) {
+  void *base = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (C.device >= 0 && C.device < NOALLOC_MAX_DEVICES) base = g_scratch_ptr[C.device];
+  }
+  if (!base) throw Error{74, "scratch array of the allocation-free entry points is not set up on this device"};
+  if (C.nao > NOALLOC_MAX_NAO)
+    throw Error{73, "allocation-free AO reordering supports at most " + std::to_string(NOALLOC_MAX_NAO) + " AOs"};
+  std::vector<long long> map;
+  std::vector<double> phase;
+  ao_map(C, ordering, map, phase);
+  if (ldx < C.nao || ldy < C.nao) throw Error{52, "Invalid number of atomic orbitals in CA/CB"};
+  if (ncols <= 0) return;
+  long long *dmap = static_cast<long long *>(base);
+  double *dph = reinterpret_cast<double *>(static_cast<unsigned char *>(base) + NOALLOC_MAX_NAO * sizeof(long long));
+  GI_HIP(hipMemcpyAsync(dmap, map.data(), sizeof(long long) * C.nao, hipMemcpyHostToDevice, C.stream));
+  GI_HIP(hipMemcpyAsync(dph, phase.data(), sizeof(double) * C.nao, hipMemcpyHostToDevice, C.stream));
+  const long long total = (long long)C.nao * ncols;
+  const unsigned grid = (unsigned)std::min<long long>((total + 255) / 256, 65536);
+  hipLaunchKernelGGL(k_ao_rows, dim3(grid), dim3(256), 0, C.stream, X, ldx, (long long)C.nao, ncols, dmap, dph, Y,
+                     ldy);
   GI_HIP(hipGetLastError());
   GI_HIP(hipStreamSynchronize(C.stream));
 }

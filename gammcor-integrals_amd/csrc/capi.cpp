@@ -49,6 +49,8 @@ int gi_create(gi_context **out, int64_t device) {
     auto *c = new gi_context;
     try {
       c->c = std::make_unique<gi::Context>((int)device);
+      GI_HIP(hipSetDevice((int)device));
+      gi::noalloc_setup((int)device);
     } catch (...) {
       delete c;
       throw;
@@ -76,8 +78,6 @@ int gi_set_basis(gi_context *cp, int64_t natoms, const double *coords, int64_t n
       if (nprim[p] < 1 || nprim[p] > max_nprim) throw Error{8, "invalid number of primitives"};
       lmax = std::max<int>(lmax, (int)momentum[p]);
     }
-    if (lmax > 3)
-      throw Error{9, "no ERI kernels for l > 3 in this build (Auto2e supports l <= 5)"};
     C.natoms = (int)natoms;
     C.nshells = (int)nshells;
     C.spher = spher ? 1 : 0;
@@ -485,6 +485,15 @@ int gi_ao_rows_to_auto2e_device(gi_context *cp, int64_t ordering, const double *
     gi::Context &C = ctx_of(cp);
     if (X_ext == X_ao) throw Error{72, "in-place AO reordering is not supported"};
     gi::ao_rows_to_auto2e(C, (int)ordering, X_ext, ld_ext, ncols, X_ao, ld_ao);
+  })
+}
+
+int gi_ao_rows_to_auto2e_device_noalloc(gi_context *cp, int64_t ordering, const double *X_ext, int64_t ld_ext,
+                                        int64_t ncols, double *X_ao, int64_t ld_ao) {
+  GI_TRY({
+    gi::Context &C = ctx_of(cp);
+    if (X_ext == X_ao) throw Error{72, "in-place AO reordering is not supported"};
+    gi::ao_rows_to_auto2e_noalloc(C, (int)ordering, X_ext, ld_ext, ncols, X_ao, ld_ao);
   })
 }
 

@@ -10,6 +10,7 @@
 #include <map>
 #include <unordered_map>
 #include "gi_context.h"
+#include "md_generic.h"
 #include "grad_kernels.h"
 #include "onee_kernels.h"
 
@@ -163,9 +164,25 @@ static void init_md_table() {
   done = true;
 }
 
+// one (bra pair, ket pair) quartet per workgroup of md_generic_kernel
+static void launch_generic(int E, int F, MdArgs a, hipStream_t s) {
+  if (E > GEN_EMAX || F > GEN_EMAX || E + F > BOYS_MAXM)
+    throw Error{60, "no ERI kernel for Hermite orders " + std::to_string(E) + ", " + std::to_string(F)};
+  a.kt = 1;
+  a.trans = 0;
+  a.ntiles = (a.mode == LAUNCH_PRODUCT) ? a.nrows * a.ncols : a.nrows;
+  const long long grid = std::max<long long>(1, std::min<long long>(a.ntiles, 16384));
+  hipLaunchKernelGGL(md_generic_kernel, dim3((unsigned)grid), dim3(GEN_NT), 0, s, a, E, F);
+}
+
 void launch_md_class(int E, int F, const MdArgs &a, int grid, hipStream_t s) {
   init_md_table();
-  if (E < 0 || E > MD_EMAX || F < 0 || F > MD_EMAX || !md_table[E][F])
+  if (E < 0 || F < 0) throw Error{60, "no ERI kernel for negative Hermite orders"};
+  if (E > MD_EMAX || F > MD_EMAX) {
+    launch_generic(E, F, a, s);
+    return;
+  }
+  if (!md_table[E][F])
     throw Error{60, "no ERI kernel for Hermite orders " + std::to_string(E) + ", " + std::to_string(F)};
   md_table[E][F](a, grid, s);
 }
@@ -226,9 +243,9 @@ void eri_run(Context &C, const EriRequest &R) {
       return;
     }
     const int l0 = C.shell_l[d.s0], l1 = C.shell_l[d.s1];
-    if (l0 + l1 > MD_EMAX) {
+    if (l0 + l1 > 2 * MAXL) {
       char buf[160];
-      snprintf(buf, sizeof buf, "no ERI kernel for a shell pair with l0 + l1 = %d (> %d)", l0 + l1, MD_EMAX);
+      snprintf(buf, sizeof buf, "no ERI kernel for a shell pair with l0 + l1 = %d (> %d)", l0 + l1, 2 * MAXL);
       throw Error{3, buf};
     }
     const int n0 = C.shell_nang[d.s0], n1 = C.shell_nang[d.s1];
@@ -340,7 +357,10 @@ void eri_run(Context &C, const EriRequest &R) {
     // a product with E > F runs transposed on the (F, E) kernel
     bool trans = false;
     bool lane_path = false;
-    if (C.eri_lane && std::max(L.E, L.F) <= C.eri_lane_emax && L.E <= MD_EMAX && L.F <= MD_EMAX) {
+    // classes beyond the templated kernels (a pair with l0 + l1 > 6) take the
+    // runtime-order kernel, one quartet per workgroup, never transposed
+    const bool generic = std::max(L.E, L.F) > MD_EMAX;
+    if (!generic && C.eri_lane && std::max(L.E, L.F) <= C.eri_lane_emax) {
       const bool lt = R.mode == LAUNCH_PRODUCT && C.eri_trans && L.E > L.F && lane_table[L.F][L.E] != nullptr;
       lane_path = lt || lane_table[L.E][L.F] != nullptr;
       trans = lt;
@@ -350,7 +370,7 @@ void eri_run(Context &C, const EriRequest &R) {
     if (!lane_path) trans = false;
     // MFMA kernel orientation: its bra side (stage-2 rows, 16-row tiles) is
     // the side with more rows per pair; the thin side is stacked in columns
-    if (!lane_path && R.mode == LAUNCH_PRODUCT && C.eri_trans) {
+    if (!lane_path && !generic && R.mode == LAUNCH_PRODUCT && C.eri_trans) {
       double sr = 0, sc = 0;
       for (int r : L.rows) sr += nrow_of(rd[r]);
       for (int c : L.cols) sc += nrow_of(cd[c]);
@@ -436,7 +456,10 @@ void eri_run(Context &C, const EriRequest &R) {
       GI_HIP(hipEventCreate(&s1));
       GI_HIP(hipEventRecord(s0, C.stream));
     }
-    (lane_path ? lane_table[KE][KF] : md_table[KE][KF])(a, (int)grid, C.stream);
+    if (generic)
+      launch_generic(KE, KF, a, C.stream);
+    else
+      (lane_path ? lane_table[KE][KF] : md_table[KE][KF])(a, (int)grid, C.stream);
     GI_HIP(hipGetLastError());
     ++nlaunch;
     if (stats) {

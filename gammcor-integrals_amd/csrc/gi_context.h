@@ -176,6 +176,10 @@ void ao_map(const Context &C, int ordering, std::vector<long long> &map, std::ve
 // rows of X (nao x ncols, device) to Auto2e order with the phases applied
 void ao_rows_to_auto2e(Context &C, int ordering, const double *X, long long ldx, long long ncols, double *Y,
                        long long ldy);
+// allocation-free variant: map and signs staged in a static device array
+void ao_rows_to_auto2e_noalloc(Context &C, int ordering, const double *X, long long ldx, long long ncols, double *Y,
+                               long long ldy);
+void noalloc_setup(int device);   // with `device` current; once per device
 
 // frees the context's gi_chol_build work arrays (chol.hip)
 void release_chol_work(Context &C);

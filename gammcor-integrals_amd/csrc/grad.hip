@@ -49,13 +49,11 @@ void coulomb_energy_grad(Context &C, const double *D, long long ldd, double kapp
   // shell pairs s0 >= s1, grouped by E = l0 + l1 so that a class is a contiguous range
   const int nbin = 2 * MAXL + 1;
   std::vector<std::vector<std::pair<int, int>>> bins(nbin);
+  // R tables of order (E + 1) + F for the gradient: within the Boys table
+  if (want_grad && 4 * C.lmax + 1 > BOYS_MAXM)
+    throw Error{62, "the nuclear gradient needs 4 lmax + 1 <= " + std::to_string(BOYS_MAXM) + " (l <= 4)"};
   for (int s1 = 0; s1 < nsh; ++s1)
-    for (int s0 = s1; s0 < nsh; ++s0) {
-      const int E = C.shell_l[s0] + C.shell_l[s1];
-      if (want_grad && E + 1 > MD_EMAX)
-        throw Error{62, "the nuclear gradient needs l0 + l1 <= " + std::to_string(MD_EMAX - 1)};
-      bins[E].push_back({s0, s1});
-    }
+    for (int s0 = s1; s0 < nsh; ++s0) bins[C.shell_l[s0] + C.shell_l[s1]].push_back({s0, s1});
   std::vector<long long> bin_off(nbin + 1, 0);
   for (int e = 0; e < nbin; ++e) bin_off[e + 1] = bin_off[e] + (long long)bins[e].size();
   const long long npair = bin_off[nbin];

@@ -182,6 +182,13 @@ module gigpu_c
       type(c_ptr), value :: ctx, X_ext, X_ao
       integer(c_int64_t), value :: ordering, ld_ext, ncols, ld_ao
     end function
+    ! allocation-free variant (map and signs in the library's static device array; NAO <= 65536)
+    integer(c_int) function gi_ao_rows_to_auto2e_device_noalloc(ctx, ordering, X_ext, ld_ext, ncols, X_ao, ld_ao) &
+        bind(C, name="gi_ao_rows_to_auto2e_device_noalloc")
+      import :: c_int, c_ptr, c_int64_t
+      type(c_ptr), value :: ctx, X_ext, X_ao
+      integer(c_int64_t), value :: ordering, ld_ext, ncols, ld_ao
+    end function
     integer(c_int) function gi_mo_transform(ctx, CA, lda, a0, a1, CB, ldb, b0, b1, max_buffer_mb, flags, Rkab) &
         bind(C, name="gi_mo_transform")
       import :: c_int, c_ptr, c_int64_t, c_double

@@ -18,7 +18,8 @@ EXPORTS = ["gi_create", "gi_destroy", "gi_last_error", "gi_set_basis", "gi_eri_q
            "gi_chol_rkpq_device", "gi_chol_inv_l", "gi_chol_vdiag", "gi_chol_timings",
            "gi_mo_transform", "gi_mo_transform_device", "gi_eri_class_stats", "gi_eri_pair_block",
            "gi_set_option", "gi_chol_build_batch", "gi_coulomb_energy_grad",
-           "gi_chol_residual", "gi_one_electron", "gi_h0", "gi_ao_map", "gi_ao_rows_to_auto2e_device"]
+           "gi_chol_residual", "gi_one_electron", "gi_h0", "gi_ao_map", "gi_ao_rows_to_auto2e_device",
+           "gi_ao_rows_to_auto2e_device_noalloc"]
 
 CHOL_DECODE_FIXED = 1
 CHOL_SKIP_VECTORS = 2
@@ -77,6 +78,7 @@ def load():
                                          i64, C.c_void_p]
     L.gi_ao_map.argtypes = [C.c_void_p, i64, iptr, dptr]
     L.gi_ao_rows_to_auto2e_device.argtypes = [C.c_void_p, i64, C.c_void_p, i64, i64, C.c_void_p, i64]
+    L.gi_ao_rows_to_auto2e_device_noalloc.argtypes = [C.c_void_p, i64, C.c_void_p, i64, i64, C.c_void_p, i64]
     for name in EXPORTS:
         getattr(L, name).restype = C.c_int if name != "gi_last_error" else C.c_char_p
     _lib = L
@@ -311,6 +313,12 @@ class Context:
     def ao_rows_to_auto2e_device(self, ordering, X_ptr, ld_ext, ncols, Y_ptr, ld_ao):
         _check(self.L.gi_ao_rows_to_auto2e_device(self.h, ordering, C.c_void_p(X_ptr), ld_ext, ncols,
                                                   C.c_void_p(Y_ptr), ld_ao))
+
+    def ao_rows_to_auto2e_device_noalloc(self, ordering, X_ptr, ld_ext, ncols, Y_ptr, ld_ao):
+        """gi_ao_rows_to_auto2e_device_noalloc: the same transform without device
+        allocations (map and signs staged in the library's static array)."""
+        _check(self.L.gi_ao_rows_to_auto2e_device_noalloc(self.h, ordering, C.c_void_p(X_ptr), ld_ext, ncols,
+                                                          C.c_void_p(Y_ptr), ld_ao))
 
 
 def accuracy_tau(accuracy: int) -> float:

@@ -25,6 +25,8 @@ L_LETTERS = "SPDFGH"
 BASIS_SPECS = {
     "small": {"O": (6, 4, 2), "H": (3, 1)},          # O 6s4p2d, H 3s1p (l <= 2)
     "large": {"O": (8, 5, 3, 1), "H": (4, 2, 1)},    # O 8s5p3d1f, H 4s2p1d (l <= 3)
+    # g and h shells (the reference's AUTO2E_MAXL = 5): O 4s3p2d1f1g1h, H 3s2p1d1f1g
+    "gh": {"O": (4, 3, 2, 1, 1, 1), "H": (3, 2, 1, 1, 1)},
 }
 
 

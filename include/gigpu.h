@@ -35,10 +35,12 @@ typedef struct gi_context gi_context;
                                    (TwoStepCholesky_Step2.f90:520-530) */
 #define GI_CHOL_SKIP_VECTORS 2  /* stop after Step 2 (no Rkpq) */
 #define GI_CHOL_TIME_ERI 4      /* time ERI launches with HIP events */
-#define GI_CHOL_RAW_TIES 8      /* compare the sort keys and the batch argmax on
-                                   raw doubles, as the reference does, instead of
+#define GI_CHOL_RAW_TIES 8      /* order the sorts and the batch argmax by raw
+                                   doubles, as the reference does, instead of by
                                    keys quantized to 2^-26 relative (BASE sort) and
-                                   2^-32 of max(Vdiag) (residual diagonals) */
+                                   to min(2^-32 of max(Vdiag), 2^-8 of tau)
+                                   (residual diagonals); thresholds (tau, Dmin)
+                                   always compare raw values */
 #define GI_CHOL_SPLIT_BF16 16   /* precision form: the Step-1 trailing update
                                    (chol2_SubtractR) as split-BF16 MFMA products
                                    (four 8-bit slices per element, exact FP32
@@ -239,6 +241,15 @@ int gi_ao_map(gi_context *ctx, int64_t ordering, int64_t *map, double *phase);
  * gi_mo_transform_device. */
 int gi_ao_rows_to_auto2e_device(gi_context *ctx, int64_t ordering, const double *X_ext, int64_t ld_ext,
                                 int64_t ncols, double *X_ao, int64_t ld_ao);
+/* Allocation-free variant of gi_ao_rows_to_auto2e_device (no counterpart in
+ * the reference; for callers that cannot allocate device memory: code under
+ * HIP graph capture, tight loops that forbid hipMalloc).  Same arguments and
+ * result; the map and signs are staged in a 1 MiB device array that the
+ * library holds per device for the life of the process instead of device
+ * allocations, so the basis may have at most 65536 AOs (larger fails).  Calls
+ * that overlap on one device race for that array. */
+int gi_ao_rows_to_auto2e_device_noalloc(gi_context *ctx, int64_t ordering, const double *X_ext, int64_t ld_ext,
+                                        int64_t ncols, double *X_ao, int64_t ld_ao);
 
 #ifdef __cplusplus
 }

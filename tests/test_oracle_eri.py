@@ -49,6 +49,24 @@ def test_against_obara_saika(water):
 
 
 @pytest.mark.parametrize("spher", [True, False], ids=["spherical", "cartesian"])
+def test_g_and_h_shells_against_obara_saika(oracle_mod, spher):
+    """Classes with g and h shells (AUTO2E_MAXL = 5) against the independent
+    evaluator, spherical and Cartesian."""
+    b = systems.water_system(1, 6.0, "gh", seed=3, spher_ao=spher)
+    o = oracle_mod.Oracle(b)
+    rng = np.random.default_rng(7)
+    by_l = {l: [i for i in range(b.nshells) if b.shell_l(i) == l] for l in range(6)}
+    combos = [(4, 0, 0, 0), (4, 1, 2, 0), (5, 0, 1, 0), (4, 4, 0, 0), (5, 4, 2, 1), (4, 3, 4, 2), (5, 5, 1, 0),
+              (0, 5, 3, 3), (5, 2, 5, 0)]
+    for ls in combos:
+        q = [int(rng.choice(by_l[l])) for l in ls]
+        V = o.eri(*q)
+        R = ind.shell_quartet(b, q)
+        err = np.abs(V - R).max()
+        assert err <= 2e-11 * max(1.0, np.abs(R).max()), (ls, q, err)
+
+
+@pytest.mark.parametrize("spher", [True, False], ids=["spherical", "cartesian"])
 def test_contracted_shells_against_obara_saika(oracle_mod, spher):
     """Segmented contractions (3- and 2-primitive shells mixed with single
     primitives) in spherical and Cartesian AOs against the independent
