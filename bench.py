@@ -210,9 +210,12 @@ def gradient_form(reps: int = 3):
 
 
 def cpu_baseline(seed: int):
+    """The CPU restatement (oracle, C/OpenMP) on a bounded sample -- the full
+    build of (H2O)5 with the headline basis -- and, beside it, the GPU build
+    of the same system, so that the two rates are on one workload."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
-    from gigpu import systems
+    from gigpu import lib, systems
     pyoracle.build()
     b = systems.water_system(5, 10.0, "large", seed=seed)
     o = pyoracle.Oracle(b)
@@ -220,9 +223,53 @@ def cpu_baseline(seed: int):
     r = o.chol(tau=1e-6)
     dt = time.perf_counter() - t0
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    ctx = lib.Context(0)
+    try:
+        ctx.set_basis(b)
+        ctx.chol_build(tau=1e-6)
+        ts = []
+        for _ in range(3):
+            t1 = time.perf_counter()
+            ctx.set_basis(b)
+            ctx.chol_build(tau=1e-6)
+            ts.append(time.perf_counter() - t1)
+        gd = ctx.chol_dims()
+        same_pivots = bool(np.array_equal(ctx.chol_meta()["pivot_order"], r.pivot_order))
+    finally:
+        ctx.close()
+    tg = float(np.median(ts))
     return {"value": r.nquartets / dt, "unit": "quartets/s", "cores": threads, "kind": "port",
-            "sample": "CPU oracle (C/OpenMP restatement), full two-step Cholesky of (H2O)5 with the same "
-                      "basis (%d AOs, %d vectors, %d quartets) in %.2f s" % (b.nao, r.nvecs, r.nquartets, dt)}
+            "sample": "CPU restatement (oracle, C/OpenMP), full two-step Cholesky of (H2O)5 with the same "
+                      "basis (%d AOs, %d vectors, %d quartets) in %.2f s" % (b.nao, r.nvecs, r.nquartets, dt),
+            "gpu_same_sample": {"build_s": tg, "quartets_per_s": gd["nquartets"] / tg, "nvecs": gd["nvecs"],
+                                "same_pivot_order": same_pivots, "speedup_vs_cpu": dt / tg}}
+
+
+def decode_and_ties(ctx, basis, ref_order):
+    """Beside the headline build (the reference's chol2_pq2p_ge_q decode of
+    same-shell pivots in Step 2, reproduced for parity): the build with the
+    Step-1 decode (GI_CHOL_DECODE_FIXED), whose reconstruction error is the
+    bound tau, and the build with raw (unquantized) comparison keys
+    (GI_CHOL_RAW_TIES), compared pivot by pivot with the headline build."""
+    from gigpu import lib
+    out = {}
+    ctx.set_basis(basis)
+    ctx.chol_build(tau=1e-6, flags=lib.CHOL_DECODE_FIXED)
+    t0 = time.perf_counter()
+    ctx.set_basis(basis)
+    ctx.chol_build(tau=1e-6, flags=lib.CHOL_DECODE_FIXED)
+    tb = time.perf_counter() - t0
+    mx, mn = ctx.chol_residual()
+    out["fixed_decode"] = {"build_s": tb, "nvecs": ctx.chol_dims()["nvecs"], "residual_max": mx,
+                           "residual_min": mn}
+    ctx.chol_build(tau=1e-6, flags=lib.CHOL_RAW_TIES | lib.CHOL_SKIP_VECTORS)
+    raw = ctx.chol_meta()["pivot_order"]
+    n = min(len(raw), len(ref_order))
+    out["raw_ties"] = {"nvecs": int(len(raw)), "same_pivot_order": bool(np.array_equal(raw, ref_order)),
+                       "first_difference": (int(np.argmax(raw[:n] != ref_order[:n]))
+                                            if np.any(raw[:n] != ref_order[:n]) else None),
+                       "positions_differing": int(np.count_nonzero(raw[:n] != ref_order[:n]) + abs(len(raw) - len(ref_order)))}
+    return out
 
 
 def main():
@@ -239,6 +286,8 @@ def main():
     ap.add_argument("--no-autotune", action="store_true", help="skip autotuning the batched form")
     ap.add_argument("--no-gradient", action="store_true", help="skip the differentiation-form timing")
     ap.add_argument("--no-precision", action="store_true", help="skip the split-BF16 precision-form comparison")
+    ap.add_argument("--no-decode-ties", action="store_true",
+                    help="skip the fixed-decode and raw-tie builds beside the headline build")
     ap.add_argument("--eri-stats", action="store_true",
                     help="print per-class ERI kernel statistics to stderr (sets GIGPU_ERI_STATS)")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
@@ -362,6 +411,14 @@ def main():
                      "flops": eri_flops, "kernel_s": eri_s},
     }
     stat_rows = sorted(ctx.eri_class_stats(), key=lambda r: -r[3]) if args.eri_stats else []
+    # reconstruction error of the headline build (outside the timed region)
+    res_max, res_min = ctx.chol_residual()
+    head_order = ctx.chol_meta()["pivot_order"]
+    out["decode"] = "reference chol2_pq2p_ge_q (TwoStepCholesky_Step2.f90:422-466), reproduced for parity"
+    out["residual_max"] = res_max
+    out["residual_min"] = res_min
+    if not args.no_decode_ties:
+        out.update(decode_and_ties(ctx, basis, head_order))
     if not args.no_precision:
         out["precision"] = precision_form(ctx, basis, Cmo, hip)
     ctx.close()

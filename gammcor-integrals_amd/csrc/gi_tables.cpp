@@ -1,9 +1,10 @@
 // Host-side tables for the device ERI kernels.
 //
 // Boys table: 5-point Chebyshev coefficients of F_m on 500 intervals of
-// [0, 25) for m = 0..maxm, built from a reference evaluation of F_m at the
-// Chebyshev nodes of every interval (boys.f90:57-86 with fm_reference
-// boys.f90:94-199 and cheb_nodes/cheb_coeffs chebinterp.f90:9-56).  The device
+// [0, 25) for m = 0..maxm, built from an accurate evaluation of F_m at the
+// Chebyshev nodes of every interval (the table layout, nodes and coefficient
+// formula of boys.f90:57-86 and cheb_nodes/cheb_coeffs chebinterp.f90:9-56;
+// F_m at the nodes by the series below).  The device
 // interpolates only at the top order a block needs and recurses downwards,
 // exactly as the reference's fm() does, so ERIs agree with the reference's
 // tabulation error included.
@@ -18,42 +19,19 @@ namespace gi {
 
 static const double PI_ = 3.141592653589793;
 
-static double shavitt_tail(long m, double twox, double expmx) {
-  // Shavitt's series, six terms: sum_i (2x)^i / ((2m+1)(2m+3)...(2m+2i+1))
-  double sum = 0.0, xi = 1.0;
-  long a = 2 * m + 1, d = a;
-  for (int i = 0; i < 6; ++i) {
-    sum = sum + xi / (double)d;
-    xi = xi * twox;
-    a += 2;
-    d = d * a;
-  }
-  return expmx * sum;
-}
-
+// F_m(x) at the table's Chebyshev nodes, x in [0, 25): the series
+//   F_m(x) = exp(-x) sum_{i>=0} (2x)^i / ((2m+1)(2m+3)...(2m+2i+1))
+// (every term positive, so no cancellation), summed in extended precision
+// until a term falls below 2^-70 of the sum.  Accurate to the last bit of a
+// double over the whole table range.
 static double boys_accurate(long m, double x) {
-  double f;
-  if (x < 35.0) {
-    // start high above m and recurse down (stable direction)
-    long dm = x < 1.0 ? 15 : x < 5.0 ? 30 : x < 10.0 ? 50 : x < 15.0 ? 60
-            : x < 20.0 ? 70 : x < 25.0 ? 80 : x < 30.0 ? 90 : 100;
-    double expmx = std::exp(-x), twox = 2.0 * x;
-    f = shavitt_tail(m + dm, twox, expmx);
-    long a = 2 * (m + dm - 1) + 1;
-    for (long l = 1; l <= dm; ++l) {
-      f = 1.0 / (double)a * (twox * f + expmx);
-      a -= 2;
-    }
-  } else {
-    f = 0.5 * std::sqrt(PI_ / x);
-    double expmx = std::exp(-x), beta = 1.0 / (2.0 * x);
-    long a = 1;
-    for (long l = 2; l <= m + 1; ++l) {
-      f = beta * ((double)a * f - expmx);
-      a += 2;
-    }
+  const long double twox = 2.0L * (long double)x;
+  long double term = 1.0L / (long double)(2 * m + 1), sum = term;
+  for (long i = 1; i < 100000 && term > sum * 0x1p-70L; ++i) {
+    term *= twox / (long double)(2 * m + 2 * i + 1);
+    sum += term;
   }
-  return f;
+  return (double)(std::exp(-(long double)x) * sum);
 }
 
 void boys_table_host(int maxm, std::vector<double> &table) {

@@ -235,8 +235,9 @@ module gigpu_cholesky
   use gigpu_c
   implicit none
 
-  ! TChol2Vecs index arrays (TwoStepCholesky_definitions.f90:56-74) and Rkpq
-  ! (TCholeskyVecsOTF, Cholesky_Gammcor.f90:19-22) of one decomposition
+  ! TChol2Vecs (TwoStepCholesky_definitions.f90:56-74: dimensions, index
+  ! arrays and Inv_L) and Rkpq (TCholeskyVecsOTF, Cholesky_Gammcor.f90:19-22)
+  ! of one decomposition
   type gigpu_chol_vecs
     type(c_ptr) :: ctx = c_null_ptr
     integer(c_int64_t) :: NVecs = 0, NOrbPairs = 0, NShellPairs = 0, NSubsets = 0, MaxSubsetDim = 0
@@ -245,6 +246,7 @@ module gigpu_cholesky
     integer(c_int64_t), allocatable :: SubsetDim(:), SubsetBounds(:, :), Pivots(:)
     integer(c_int64_t), allocatable :: PivotShellPairs(:), PivotShellPairLoc(:), PivotShellPairDim(:)
     integer(c_int64_t), allocatable :: PivotOrbPairs(:)
+    real(c_double), allocatable :: Inv_L(:, :)
     real(c_double), allocatable :: Rkpq(:, :, :)
   end type
 
@@ -281,7 +283,7 @@ contains
     vecs%NPivotShellPairs = dims(6)
     if (allocated(vecs%ShellPairs)) deallocate(vecs%ShellPairs, vecs%ShellPairLoc, vecs%ShellPairDim, &
         vecs%SubsetDim, vecs%SubsetBounds, vecs%Pivots, vecs%PivotShellPairs, vecs%PivotShellPairLoc, &
-        vecs%PivotShellPairDim, vecs%PivotOrbPairs, vecs%Rkpq)
+        vecs%PivotShellPairDim, vecs%PivotOrbPairs, vecs%Inv_L, vecs%Rkpq)
     allocate(vecs%ShellPairs(2, vecs%NShellPairs), vecs%ShellPairLoc(3, vecs%NShellPairs), &
         vecs%ShellPairDim(vecs%NShellPairs), vecs%SubsetDim(vecs%NSubsets), vecs%SubsetBounds(2, vecs%NSubsets), &
         vecs%Pivots(vecs%NOrbPairs), vecs%PivotShellPairs(vecs%NPivotShellPairs), &
@@ -290,6 +292,8 @@ contains
     call gigpu_check(gi_chol_meta(vecs%ctx, vecs%ShellPairs, vecs%ShellPairLoc, vecs%ShellPairDim, &
         vecs%SubsetDim, vecs%SubsetBounds, vecs%Pivots, pivot_order, vecs%PivotShellPairs, &
         vecs%PivotShellPairLoc, vecs%PivotShellPairDim, vecs%PivotOrbPairs))
+    allocate(vecs%Inv_L(vecs%NVecs, vecs%NVecs))
+    call gigpu_check(gi_chol_inv_l(vecs%ctx, vecs%Inv_L))
     allocate(vecs%Rkpq(vecs%NVecs, vecs%MaxSubsetDim, vecs%NSubsets))
     call gigpu_check(gi_chol_rkpq(vecs%ctx, vecs%Rkpq))
   end subroutine

@@ -66,3 +66,85 @@ double or_coulomb_energy(const or_basis *b, const double *D, int64_t ldd, double
     free(pb);
     return 0.5 * E;
 }
+
+/* Nuclear gradient of E_J from derivative ERIs (or_eri_deriv_a, eri.c):
+ *   dE_J/dR_A = 2 sum_{a on A} sum_b sum_{cd} D_ab D_cd (d_A a b|cd),
+ * E_J being symmetric in the two pairs and D symmetric, so only the first
+ * function of the bra is differentiated.  Every ordered bra shell pair whose
+ * first shell sits on an atom of `atoms` (all atoms when natoms_sel < 0),
+ * ket shell pairs sc >= sd weighted 2 when sc != sd.  G is (3, natoms),
+ * zeroed here; atoms outside the selection stay zero.  Returns non-zero if
+ * a derivative class is beyond the oracle's tables. */
+int or_coulomb_grad(const or_basis *b, const double *D, int64_t ldd, double kappa, int64_t natoms_sel,
+                    const int64_t *atoms, double *G)
+{
+    const int64_t nsh = b->nshells;
+    int64_t *loc = malloc(sizeof(int64_t) * (size_t)nsh);
+    int64_t nmax = 1;
+    for (int64_t s = 0, l = 0; s < nsh; ++s) {
+        loc[s] = l;
+        const int64_t n = or_nang(b, s);
+        l += n;
+        if (n > nmax) nmax = n;
+    }
+    char *want = calloc((size_t)b->natoms, 1);
+    for (int64_t i = 0; i < b->natoms; ++i) want[i] = natoms_sel < 0;
+    for (int64_t i = 0; i < natoms_sel; ++i)
+        if (atoms[i] >= 0 && atoms[i] < b->natoms) want[atoms[i]] = 1;
+    for (int64_t i = 0; i < 3 * b->natoms; ++i) G[i] = 0.0;
+    int64_t nbra = 0;
+    int64_t *ba = malloc(sizeof(int64_t) * (size_t)(nsh * nsh)), *bb = malloc(sizeof(int64_t) * (size_t)(nsh * nsh));
+    for (int64_t sa = 0; sa < nsh; ++sa)
+        if (want[b->shell_centers[sa] - 1])
+            for (int64_t sb = 0; sb < nsh; ++sb) {
+                ba[nbra] = sa;
+                bb[nbra] = sb;
+                ++nbra;
+            }
+    int status = 0;
+#pragma omp parallel
+    {
+        double *dV = malloc(sizeof(double) * (size_t)(3 * nmax * nmax * nmax * nmax));
+        double *Gl = calloc((size_t)(3 * b->natoms), sizeof(double));
+#pragma omp for schedule(dynamic, 1)
+        for (int64_t k = 0; k < nbra; ++k) {
+            const int64_t sa = ba[k], sb = bb[k];
+            const int64_t na = or_nang(b, sa), nbn = or_nang(b, sb);
+            double g[3] = {0.0, 0.0, 0.0};
+            for (int64_t sc = 0; sc < nsh; ++sc)
+                for (int64_t sd = 0; sd <= sc; ++sd) {
+                    const int64_t nc = or_nang(b, sc), nd = or_nang(b, sd);
+                    if (or_eri_deriv_a(b, sa, sb, sc, sd, kappa, dV) != 0) {
+#pragma omp atomic write
+                        status = 1;
+                        continue;
+                    }
+                    const int64_t nq = na * nbn * nc * nd;
+                    const double w = sc != sd ? 2.0 : 1.0;
+                    for (int dir = 0; dir < 3; ++dir) {
+                        double s = 0.0;
+                        for (int64_t d = 0; d < nd; ++d)
+                            for (int64_t c = 0; c < nc; ++c) {
+                                const double dcd = D[(loc[sc] + c) + ldd * (loc[sd] + d)];
+                                for (int64_t q = 0; q < nbn; ++q)
+                                    for (int64_t a = 0; a < na; ++a)
+                                        s += D[(loc[sa] + a) + ldd * (loc[sb] + q)] * dcd *
+                                             dV[dir * nq + a + na * (q + nbn * (c + nc * d))];
+                            }
+                        g[dir] += w * s;
+                    }
+                }
+            const int64_t at = b->shell_centers[sa] - 1;
+            for (int dir = 0; dir < 3; ++dir) Gl[3 * at + dir] += 2.0 * g[dir];
+        }
+#pragma omp critical
+        for (int64_t i = 0; i < 3 * b->natoms; ++i) G[i] += Gl[i];
+        free(Gl);
+        free(dV);
+    }
+    free(ba);
+    free(bb);
+    free(want);
+    free(loc);
+    return status;
+}

@@ -463,3 +463,110 @@ void or_eri_pair_matrix(const or_basis *b, int64_t npairs, const int64_t *pairs,
     }
     free(off);
 }
+
+/* ---------------- derivative integrals ----------------
+ * Not part of the reference (it has no gradients): the checker of the
+ * library's nuclear gradient.  The derivative of a Cartesian Gaussian with
+ * respect to its own centre is a combination of Gaussians one order up and
+ * one order down on the same centre,
+ *   d/dA_x [x_A^lx y_A^ly z_A^lz exp(-a r_A^2)]
+ *       = 2a x_A^(lx+1) y_A^ly z_A^lz exp(-a r_A^2) - lx x_A^(lx-1) y_A^ly z_A^lz exp(-a r_A^2),
+ * so (d_A a b|cd) comes from two ordinary Cartesian ERI blocks -- shell a
+ * raised to la + 1 with coefficients 2 a_i c_i, and lowered to la - 1 with
+ * coefficients c_i -- followed by the solid-harmonic transform and the
+ * normalization of the original shells.  No Hermite-derivative identity is
+ * used, so this is an independent route to the gradient. */
+
+/* Cartesian block of four shells (unnormalized, no spherical transform) in
+ * the caller's order H[i0 + n0 (i1 + n1 (i2 + n2 i3))], via the canonical
+ * evaluation (Canonical4Tuple, Auto2e.py:1612-1647). */
+static void eri_cart_callers(const shell_t *s0, const shell_t *s1, const shell_t *s2, const shell_t *s3,
+                             double kappa, double *V)
+{
+    const shell_t *sh[4] = {s0, s1, s2, s3};
+    int m[4] = {0, 1, 2, 3};
+    int l[4] = {s0->l, s1->l, s2->l, s3->l};
+    if (l[1] > l[0]) { int t = m[0]; m[0] = m[1]; m[1] = t; t = l[0]; l[0] = l[1]; l[1] = t; }
+    if (l[3] > l[2]) { int t = m[2]; m[2] = m[3]; m[3] = t; t = l[2]; l[2] = l[3]; l[3] = t; }
+    if (l[2] > l[0] || (l[2] == l[0] && l[3] > l[1])) {
+        int t0 = m[0], t1 = m[1]; m[0] = m[2]; m[1] = m[3]; m[2] = t0; m[3] = t1;
+        t0 = l[0]; t1 = l[1]; l[0] = l[2]; l[1] = l[3]; l[2] = t0; l[3] = t1;
+    }
+    int nn[4], no[4];
+    for (int k = 0; k < 4; ++k) nn[k] = ncart(l[k]);
+    for (int k = 0; k < 4; ++k) no[m[k]] = nn[k];
+    double *H = (double *)malloc(sizeof(double) * nn[0] * nn[1] * nn[2] * nn[3]);
+    eri_cart_canonical(sh[m[0]], sh[m[1]], sh[m[2]], sh[m[3]], kappa, 0, H);
+    int idx[4];
+    for (idx[3] = 0; idx[3] < nn[3]; ++idx[3])
+        for (idx[2] = 0; idx[2] < nn[2]; ++idx[2])
+            for (idx[1] = 0; idx[1] < nn[1]; ++idx[1])
+                for (idx[0] = 0; idx[0] < nn[0]; ++idx[0]) {
+                    int o[4];
+                    for (int k = 0; k < 4; ++k) o[m[k]] = idx[k];
+                    V[o[0] + no[0] * (o[1] + no[1] * (o[2] + no[2] * o[3]))] =
+                        H[idx[0] + nn[0] * (idx[1] + nn[1] * (idx[2] + nn[2] * idx[3]))];
+                }
+    free(H);
+}
+
+int or_eri_deriv_a(const or_basis *b, int64_t sa, int64_t sb, int64_t sc, int64_t sd, double kappa, double *dV)
+{
+    shell_t sh[4];
+    int64_t ss[4] = {sa, sb, sc, sd};
+    for (int k = 0; k < 4; ++k) get_shell(b, ss[k], &sh[k]);
+    const int la = sh[0].l;
+    if (la + 1 + sh[1].l >= EMAX || sh[2].l + sh[3].l >= EMAX || la + 1 + sh[1].l + sh[2].l + sh[3].l > 20)
+        return 1;
+    double cp[64];
+    if (sh[0].np > 64) return 2;
+    for (int i = 0; i < sh[0].np; ++i) cp[i] = 2.0 * sh[0].e[i] * sh[0].c[i];
+    shell_t up = sh[0], dn = sh[0];
+    up.l = la + 1;
+    up.c = cp;
+    dn.l = la - 1;
+    const int n0 = ncart(la), n0p = ncart(la + 1), n0m = la > 0 ? ncart(la - 1) : 0;
+    const int rest = ncart(sh[1].l) * ncart(sh[2].l) * ncart(sh[3].l);
+    double *Hp = (double *)malloc(sizeof(double) * n0p * rest);
+    double *Hm = la > 0 ? (double *)malloc(sizeof(double) * n0m * rest) : NULL;
+    eri_cart_callers(&up, &sh[1], &sh[2], &sh[3], kappa, Hp);
+    if (la > 0) eri_cart_callers(&dn, &sh[1], &sh[2], &sh[3], kappa, Hm);
+    int nout = 1;
+    for (int k = 0; k < 4; ++k) nout *= (b->spher && sh[k].l >= 2) ? 2 * sh[k].l + 1 : ncart(sh[k].l);
+    double *X = (double *)malloc(sizeof(double) * n0 * rest);
+    for (int d = 0; d < 3; ++d) {
+        for (int i = 0; i < n0; ++i) {
+            int lx, ly, lz;
+            cart_comp(la, i, &lx, &ly, &lz);
+            const int lq = d == 0 ? lx : (d == 1 ? ly : lz);
+            const int ip = cart_idx(lx + (d == 0), ly + (d == 1), lz + (d == 2));
+            const int im = lq > 0 ? cart_idx(lx - (d == 0), ly - (d == 1), lz - (d == 2)) : -1;
+            for (int r = 0; r < rest; ++r) {
+                double v = Hp[ip + n0p * r];
+                if (im >= 0) v -= (double)lq * Hm[im + n0m * r];
+                X[i + n0 * r] = v;
+            }
+        }
+        int nn[4];
+        for (int k = 0; k < 4; ++k) nn[k] = ncart(sh[k].l);
+        if (b->spher)
+            for (int k = 0; k < 4; ++k)
+                if (sh[k].l >= 2) {
+                    double U[11 * 66];
+                    or_spher_u(sh[k].l, U);
+                    transform_index(X, nn, k, U, 2 * sh[k].l + 1);
+                }
+        double *out = dV + (size_t)d * nout;
+        for (int i3 = 0; i3 < nn[3]; ++i3)
+            for (int i2 = 0; i2 < nn[2]; ++i2)
+                for (int i1 = 0; i1 < nn[1]; ++i1)
+                    for (int i0 = 0; i0 < nn[0]; ++i0) {
+                        const int q = i0 + nn[0] * (i1 + nn[1] * (i2 + nn[2] * i3));
+                        out[q] = X[q] * sh[0].nrm[i0] * sh[1].nrm[i1] * sh[2].nrm[i2] * sh[3].nrm[i3];
+                    }
+    }
+    free(X);
+    free(Hp);
+    free(Hm);
+    return 0;
+}

@@ -60,6 +60,10 @@ void or_eri_quartet(const or_basis *b, int64_t sa, int64_t sb, int64_t sc, int64
                     double kappa, double *V);
 /* Same but with one Boys evaluation (top order la+lb+lc+ld) per primitive
  * quartet; used only to measure the Boys top-order sensitivity. */
+/* d(ab|cd)/dA, A the centre of shell sa (only shell sa's functions move):
+ * dV[3][na nb nc nd], each block in or_eri_quartet's layout.  Non-zero
+ * return when la + 1 + lb or the total order is beyond the oracle's tables. */
+int or_eri_deriv_a(const or_basis *b, int64_t sa, int64_t sb, int64_t sc, int64_t sd, double kappa, double *dV);
 void or_eri_quartet_single_boys(const or_basis *b, int64_t sa, int64_t sb, int64_t sc,
                                 int64_t sd, double kappa, double *V);
 
@@ -133,6 +137,10 @@ void or_one_electron(const or_basis *b, const double *atom_charges, int64_t npc,
 /* ---- Coulomb energy of an AO density (coulomb.c) ---- */
 /* E_J = 1/2 sum D_pq (pq|rs) D_rs; D column-major (nao x nao, leading dim ldd). */
 double or_coulomb_energy(const or_basis *b, const double *D, int64_t ldd, double kappa);
+/* dE_J/dR (3, natoms) from derivative ERIs, for the atoms listed (all when
+ * natoms_sel < 0); non-zero return if a class is beyond the oracle's tables */
+int or_coulomb_grad(const or_basis *b, const double *D, int64_t ldd, double kappa, int64_t natoms_sel,
+                    const int64_t *atoms, double *G);
 
 /* ---- dense linear algebra used by the oracle (LAPACK/BLAS semantics of
  *      dpotrf "L", dtrtri "L","N" and dgemm, real_linalg.f90:39-72,1079-1361) ---- */

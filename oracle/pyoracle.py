@@ -49,7 +49,8 @@ _lib = None
 
 
 def build(force: bool = False) -> None:
-    src = [os.path.join(HERE, f) for f in ("boys.c", "eri.c", "sort.c", "linalg.c", "chol.c", "mo.c", "oracle.h")]
+    src = [os.path.join(HERE, f) for f in ("boys.c", "eri.c", "sort.c", "linalg.c", "chol.c", "mo.c", "coulomb.c",
+                                           "onee.c", "oracle.h")]
     if (not force and os.path.exists(LIB_PATH)
             and os.path.getmtime(LIB_PATH) >= max(os.path.getmtime(s) for s in src)):
         return
@@ -77,6 +78,10 @@ def lib():
                                       C.c_void_p, i64]
         L.or_coulomb_energy.argtypes = [C.POINTER(OrBasis), dptr, i64, C.c_double]
         L.or_coulomb_energy.restype = C.c_double
+        L.or_eri_deriv_a.argtypes = [C.POINTER(OrBasis), i64, i64, i64, i64, C.c_double, dptr]
+        L.or_eri_deriv_a.restype = C.c_int
+        L.or_coulomb_grad.argtypes = [C.POINTER(OrBasis), dptr, i64, C.c_double, i64, iptr, dptr]
+        L.or_coulomb_grad.restype = C.c_int
         L.or_eri_pair_matrix.argtypes = [C.POINTER(OrBasis), i64, iptr, C.c_double, dptr, i64]
         L.or_dsort0.argtypes = [dptr, iptr, i64, i64]
         L.or_chol_build.argtypes = [C.POINTER(OrBasis), C.c_double, C.c_double, i64, i64, i64,
@@ -154,6 +159,27 @@ class Oracle:
         """E_J = 1/2 sum D_pq (pq|rs) D_rs of a symmetric AO density."""
         D = np.asfortranarray(np.asarray(density, dtype=np.float64))
         return float(self.L.or_coulomb_energy(C.byref(self.ob), _p(D), D.shape[0], kappa))
+
+    def eri_deriv(self, sa, sb, sc, sd, kappa=0.0):
+        """d(ab|cd)/dA for the centre A of shell sa (0-based shells; only
+        shell sa's functions move): array [3, a, b, c, d]."""
+        n = [self.L.or_nang(C.byref(self.ob), s) for s in (sa, sb, sc, sd)]
+        dV = np.zeros(3 * n[0] * n[1] * n[2] * n[3])
+        if self.L.or_eri_deriv_a(C.byref(self.ob), sa, sb, sc, sd, kappa, _p(dV)) != 0:
+            raise ValueError("derivative class beyond the oracle's tables")
+        return dV.reshape(3, n[3], n[2], n[1], n[0]).transpose(0, 4, 3, 2, 1)
+
+    def coulomb_grad(self, density, kappa=0.0, atoms=None):
+        """dE_J/dR (natoms, 3) from derivative ERIs; with `atoms` (0-based
+        indices) only those rows are computed, the others are zero."""
+        D = np.asfortranarray(np.asarray(density, dtype=np.float64))
+        G = np.zeros(3 * self.basis.natoms)
+        sel = np.ascontiguousarray(np.asarray([] if atoms is None else atoms, dtype=np.int64))
+        nsel = -1 if atoms is None else len(sel)
+        if self.L.or_coulomb_grad(C.byref(self.ob), _p(D), D.shape[0], kappa, nsel,
+                                  _p(sel if len(sel) else np.zeros(1, np.int64), i64), _p(G)) != 0:
+            raise ValueError("derivative class beyond the oracle's tables")
+        return G.reshape(self.basis.natoms, 3)
 
     def chol(self, tau=1e-6, kappa=0.0, max_block_dim=4000, vec_block=500, flags=0):
         res = OrCholResult()

@@ -24,8 +24,8 @@ def _ref_meta(r):
                 pivot_orb_pairs=r.pivot_orb_pairs)
 
 
-def _compare(ctx, o, b, tau, flags=0, max_block_dim=4000):
-    ref = o.chol(tau=tau, flags=flags, max_block_dim=max_block_dim)
+def _compare(ctx, o, b, tau, flags=0, max_block_dim=4000, oracle_flags=None):
+    ref = o.chol(tau=tau, flags=flags if oracle_flags is None else oracle_flags, max_block_dim=max_block_dim)
     assert ref.status == 0, ref.message
     ctx.set_basis(b)
     ctx.chol_build(tau=tau, flags=flags, max_block_dim=max_block_dim)

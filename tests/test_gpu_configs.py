@@ -1,0 +1,109 @@
+"""The named workloads on the GPU, each to its parity bar.
+
+* (H2O)4 with O 6s4p2d / H 3s1p (160 AOs) and (H2O)3 with O 8s5p3d1f /
+  H 4s2p1d (225 AOs): pivot sequence, count and every TChol2Vecs array equal
+  to the oracle's; Vdiag to 1e-12; R^T R(:, P) and L L^T = V(P, P) to 1e-11
+  relative.
+* Raw comparison keys (GI_CHOL_RAW_TIES, the reference's own comparisons of
+  unrounded doubles) against the oracle in the same mode.
+* 64 independent (H2O)n, n uniform in 1..12: the concurrent batch equals the
+  one-at-a-time builds (pivot order, vector count and reconstruction error of
+  every entry; Rkpq bit for bit on the entries up to (H2O)4).
+* (H2O)16 with O 8s5p3d1f / H 4s2p1d (1200 AOs): the split-BF16 trailing
+  update gives the FP64 pivots and a reconstruction error within tau (Step-1
+  decode); the split-BF16 MO transform with nocc = 80, nvirt = 600 stays
+  within 1e-8 of max |Rkab| of the FP64 transform."""
+import numpy as np
+import pytest
+
+from gigpu import lib, systems
+from test_gpu_chol import _compare
+
+pytestmark = pytest.mark.gpu
+TAU = 1e-6
+
+
+def test_small_cluster_160_aos_matches_oracle(gpu_ctx, oracle_mod):
+    b = systems.config_small_cluster()
+    assert b.nao == 160
+    ref, m, _ = _compare(gpu_ctx, oracle_mod.Oracle(b), b, TAU)
+    print("(H2O)4: %d vectors, %d pivot shell pairs" % (m["nvecs"], m["npivot_shell_pairs"]))
+
+
+def test_three_waters_large_basis_matches_oracle(gpu_ctx, oracle_mod):
+    b = systems.water_system(3, 6.0, "large", seed=11)
+    assert b.nao == 225
+    _compare(gpu_ctx, oracle_mod.Oracle(b), b, TAU)
+
+
+@pytest.mark.parametrize("n,basis,seed", [(2, "small", 31), (1, "large", 32), (4, "small", 1)])
+def test_raw_ties_match_oracle(gpu_ctx, oracle_mod, n, basis, seed):
+    b = systems.water_system(n, 6.0, basis, seed=seed)
+    # GI_CHOL_RAW_TIES is OR_CHOL_RAW_TIES (4) in the oracle
+    _compare(gpu_ctx, oracle_mod.Oracle(b), b, TAU, flags=lib.CHOL_RAW_TIES, oracle_flags=4)
+
+
+def test_batch_of_64_equals_one_at_a_time():
+    mols = systems.batch_systems(64, seed=3)
+    ns = [b.natoms // 3 for b in mols]
+    assert min(ns) >= 1 and max(ns) <= 12
+    seq = []
+    ref_ctx = lib.Context(0)
+    try:
+        for b, n in zip(mols, ns):
+            ref_ctx.set_basis(b)
+            ref_ctx.chol_build(tau=TAU)
+            seq.append((ref_ctx.chol_meta()["pivot_order"].copy(), ref_ctx.chol_residual(),
+                        ref_ctx.chol_rkpq() if n <= 4 else None))
+    finally:
+        ref_ctx.close()
+    ctxs = [lib.Context(0) for _ in mols]
+    try:
+        for c, b in zip(ctxs, mols):
+            c.set_basis(b)
+        status = lib.chol_build_batch(ctxs, tau=TAU, nworkers=12)
+        assert list(status) == [0] * len(mols)
+        for i, (c, (order, res, rk)) in enumerate(zip(ctxs, seq)):
+            assert np.array_equal(c.chol_meta()["pivot_order"], order), i
+            assert c.chol_residual() == res, i
+            if rk is not None:
+                assert np.array_equal(c.chol_rkpq(), rk), i
+    finally:
+        for c in ctxs:
+            c.close()
+
+
+@pytest.fixture(scope="module")
+def cluster_1200():
+    b = systems.config_large_cluster()
+    assert b.nao == 1200
+    ctx = lib.Context(0)
+    ctx.set_basis(b)
+    yield ctx, b
+    ctx.close()
+
+
+def test_split_trailing_update_at_1200_aos(cluster_1200):
+    ctx, _ = cluster_1200
+    ctx.chol_build(tau=TAU, flags=lib.CHOL_DECODE_FIXED | lib.CHOL_SKIP_VECTORS)
+    order64 = ctx.chol_meta()["pivot_order"].copy()
+    ctx.chol_build(tau=TAU, flags=lib.CHOL_DECODE_FIXED | lib.CHOL_SPLIT_BF16)
+    assert np.array_equal(ctx.chol_meta()["pivot_order"], order64)
+    mx, mn = ctx.chol_residual()
+    print("split trailing update: %d vectors, residual in [%.3e, %.3e]" % (len(order64), mn, mx))
+    assert mx <= TAU * (1 + 1e-9)
+    assert mn >= -1e-10
+
+
+def test_split_mo_transform_at_1200_aos(cluster_1200):
+    ctx, b = cluster_1200
+    ctx.chol_build(tau=TAU)
+    rng = np.random.default_rng(2)
+    Cm = np.asfortranarray(np.linalg.qr(rng.normal(size=(b.nao, b.nao)))[0])
+    nocc, nvirt = 80, 600
+    r64 = ctx.mo_transform(Cm, 1, nocc, Cm, nocc + 1, nocc + nvirt)
+    r16 = ctx.mo_transform(Cm, 1, nocc, Cm, nocc + 1, nocc + nvirt, flags=lib.MO_SPLIT_BF16)
+    err = float(np.abs(r16 - r64).max())
+    rmax = float(np.abs(r64).max())
+    print("split MO transform: max abs error %.3e, relative %.3e" % (err, err / rmax))
+    assert err <= 1e-8 * rmax
