@@ -219,6 +219,8 @@ int gi_set_option(gi_context *cp, const char *name, int64_t value) {
     else if (n == "eri_trans") C.eri_trans = value != 0;
     else if (n == "eri_lane_emax") C.eri_lane_emax = std::max<int64_t>(0, value);
     else if (n == "vec_blocks") C.vec_blocks = std::max<int64_t>(1, value);
+    else if (n == "eri_streams")
+      C.eri_streams = std::min<int64_t>(gi::Context::ERI_MAX_STREAMS, std::max<int64_t>(1, value));
     else throw Error{9, "unknown option " + n};
   })
 }

@@ -343,6 +343,17 @@ void eri_run(Context &C, const EriRequest &R) {
   static const bool stats = getenv("GIGPU_ERI_STATS") != nullptr;
   long long nq_total = 0;
   double flops_total = 0;
+  // prepared class launches, dispatched after all are known (largest first,
+  // each on the stream with the least work so far)
+  struct Prepared {
+    MdArgs a;
+    long long grid;
+    int kind;   // 0 MFMA kernel, 1 lane kernel, 2 runtime-order kernel
+    int KE, KF, E, F;
+    long long nq;
+    double fl;
+  };
+  std::vector<Prepared> prep;
   auto nrow_of = [&](const PairDesc &d) {
     return (double)pair_rows_host(d, C.shell_nang[d.s0], C.shell_nang[d.s1]);
   };
@@ -450,32 +461,61 @@ void eri_run(Context &C, const EriRequest &R) {
       grid = std::min<long long>(a.ntiles, 4096);
       if (grid >= 8) grid = (grid + 7) / 8 * 8;
     }
+    prep.push_back(Prepared{a, grid, generic ? 2 : (lane_path ? 1 : 0), KE, KF, L.E, L.F, nq, fl});
+    nq_total += nq;
+    flops_total += fl;
+  }
+  // per-class statistics time every launch on its own, so they stay on the
+  // context stream; otherwise the launches are spread over the side streams
+  const int nstream = stats ? 1 : (int)std::min<long long>(C.eri_streams, (long long)prep.size());
+  if (nstream > 1) {
+    if (!C.eri_fork) {
+      GI_HIP(hipEventCreateWithFlags(&C.eri_fork, hipEventDisableTiming));
+      for (int j = 0; j < Context::ERI_MAX_STREAMS - 1; ++j) {
+        GI_HIP(hipStreamCreateWithFlags(&C.eri_side[j], hipStreamNonBlocking));
+        GI_HIP(hipEventCreateWithFlags(&C.eri_join[j], hipEventDisableTiming));
+      }
+    }
+    GI_HIP(hipEventRecord(C.eri_fork, C.stream));
+    for (int j = 1; j < nstream; ++j) GI_HIP(hipStreamWaitEvent(C.eri_side[j - 1], C.eri_fork, 0));
+  }
+  std::vector<size_t> order(prep.size());
+  for (size_t i = 0; i < order.size(); ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) { return prep[x].fl > prep[y].fl; });
+  std::vector<double> load(nstream, 0.0);
+  for (size_t oi = 0; oi < order.size(); ++oi) {
+    const Prepared &P = prep[order[oi]];
+    const int j = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+    load[j] += P.fl;
+    hipStream_t st = j == 0 ? C.stream : C.eri_side[j - 1];
     hipEvent_t s0 = nullptr, s1 = nullptr;
     if (stats) {
       GI_HIP(hipEventCreate(&s0));
       GI_HIP(hipEventCreate(&s1));
-      GI_HIP(hipEventRecord(s0, C.stream));
+      GI_HIP(hipEventRecord(s0, st));
     }
-    if (generic)
-      launch_generic(KE, KF, a, C.stream);
+    if (P.kind == 2)
+      launch_generic(P.KE, P.KF, P.a, st);
     else
-      (lane_path ? lane_table[KE][KF] : md_table[KE][KF])(a, (int)grid, C.stream);
+      (P.kind == 1 ? lane_table[P.KE][P.KF] : md_table[P.KE][P.KF])(P.a, (int)P.grid, st);
     GI_HIP(hipGetLastError());
     ++nlaunch;
     if (stats) {
-      GI_HIP(hipEventRecord(s1, C.stream));
+      GI_HIP(hipEventRecord(s1, st));
       GI_HIP(hipEventSynchronize(s1));
       float ms = 0;
       GI_HIP(hipEventElapsedTime(&ms, s0, s1));
-      auto &st = C.class_stats[L.E * 16 + L.F];
-      st[0] += (double)nq;
-      st[1] += ms;
-      st[2] += fl;
+      auto &sc = C.class_stats[P.E * 16 + P.F];
+      sc[0] += (double)P.nq;
+      sc[1] += ms;
+      sc[2] += P.fl;
       (void)hipEventDestroy(s0);
       (void)hipEventDestroy(s1);
     }
-    nq_total += nq;
-    flops_total += fl;
+  }
+  for (int j = 1; j < nstream; ++j) {
+    GI_HIP(hipEventRecord(C.eri_join[j - 1], C.eri_side[j - 1]));
+    GI_HIP(hipStreamWaitEvent(C.stream, C.eri_join[j - 1], 0));
   }
   if (C.time_eri) {
     GI_HIP(hipEventRecord(e1, C.stream));

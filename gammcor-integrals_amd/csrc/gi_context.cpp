@@ -59,6 +59,11 @@ Context::~Context() {
   // every buffer is released (stream-ordered frees on `stream`) before the
   // stream goes away
   if (stream) (void)hipStreamSynchronize(stream);
+  for (int i = 0; i < ERI_MAX_STREAMS - 1; ++i) {
+    if (eri_side[i]) (void)hipStreamDestroy(eri_side[i]);
+    if (eri_join[i]) (void)hipEventDestroy(eri_join[i]);
+  }
+  if (eri_fork) (void)hipEventDestroy(eri_fork);
   if (stream) (void)hipStreamDestroy(stream);
 }
 

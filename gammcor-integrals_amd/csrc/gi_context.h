@@ -141,6 +141,13 @@ struct Context {
   bool eri_lane = true, eri_trans = true;
   long long eri_lane_emax = LANE_EMAX;   // lane kernels only for classes with E, F <= this
   long long vec_blocks = 20;             // block rows of the triangular Inv_L Vabrs^T product
+  // class launches of one ERI request are spread over this many streams (the
+  // context stream and eri_streams - 1 side streams forked from it and joined
+  // back before the request returns); their destination blocks are disjoint
+  static constexpr int ERI_MAX_STREAMS = 4;
+  long long eri_streams = ERI_MAX_STREAMS;
+  hipStream_t eri_side[ERI_MAX_STREAMS - 1] = {};
+  hipEvent_t eri_fork = nullptr, eri_join[ERI_MAX_STREAMS - 1] = {};
 
   Context(int dev);
   ~Context();

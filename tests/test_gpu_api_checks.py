@@ -3,7 +3,8 @@
 * gi_h0 / gi_one_electron with a leading dimension larger than nao;
 * the decomposition at CHOL_ACCURACY_DEBUG (tau = 1e-10): the residual
   diagonal ends below tau, with the pivots of the oracle (the quantized
-  comparison keys order near-ties only; thresholds compare raw values);
+  comparison keys order near-ties only; thresholds compare raw values); with
+  raw comparison keys, consistent with the oracle's raw-key build;
 * gi_mo_transform in several passes when MaxBufferDimMB is small (the
   reference's buffer bound, Cholesky_Gammcor.f90:176-186) equals one pass;
 * gi_chol_build_batch rejects a non-positive or NaN tau like gi_chol_build;
@@ -15,6 +16,7 @@ import pytest
 
 from gigpu import lib, systems
 from grad_helpers import tiny_basis
+from test_gpu_configs import raw_tie_consistent
 
 pytestmark = pytest.mark.gpu
 
@@ -44,8 +46,13 @@ def test_debug_accuracy_residual_below_tau(gpu_ctx, oracle_mod, flags):
     gpu_ctx.set_basis(b)
     gpu_ctx.chol_build(tau=tau, flags=flags)
     m = gpu_ctx.chol_meta()
-    assert m["nvecs"] == ref.nvecs
-    assert np.array_equal(m["pivot_order"], ref.pivot_order)
+    if flags & lib.CHOL_RAW_TIES:
+        # raw keys: mathematically equal keys are ordered by last-bit
+        # differences between the two implementations' ERIs
+        raw_tie_consistent(gpu_ctx, oracle_mod.Oracle(b), ref, m, tau, fixed_decode=True)
+    else:
+        assert m["nvecs"] == ref.nvecs
+        assert np.array_equal(m["pivot_order"], ref.pivot_order)
     rmax, rmin = gpu_ctx.chol_residual()
     assert rmax <= tau * (1 + 1e-6)
     # (pq|pq) - sum_k R_k(pq)^2 >= 0 in exact arithmetic; the vectors pass

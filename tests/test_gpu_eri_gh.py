@@ -1,7 +1,8 @@
 """GPU parity of the ERI kernels with g and h shells (l <= 5, the reference's
 AUTO2E_MAXL, auto2e.f90:275): every canonical class up to (hh|hh) in every
 caller orientation against the oracle to 1e-12 absolute, spherical and
-Cartesian, plain and range-separated, contracted; pair blocks with the
+Cartesian; contracted, plain and range-separated, to 5e-12 (the reference's
+Boys accuracy); pair blocks with the
 same-shell triangle and selection lists through the decomposition; and a
 two-step Cholesky decomposition of a g/h basis with the oracle's pivots."""
 import numpy as np
@@ -43,6 +44,16 @@ def test_all_classes_up_to_hh(gpu_ctx, oracle_mod, spher):
     print("max |gpu - oracle| =", worst)
 
 
+# Contracted g/h quartets sum many primitive quartets whose Hermite
+# expansions cancel; the library (Hermite products, one Boys evaluation per
+# primitive quartet) and the oracle ((e0|f0) blocks and the horizontal
+# recurrence) round differently there.  Measured worst: 1.6e-12 on a (gg|gg)
+# block of magnitude 1.9e-2.  The bar is the accuracy of the reference's own
+# Boys interpolation (BOYS_ACCURACY = 3: 4.3e-12, boys.f90), which bounds
+# what any implementation can be pinned to.
+TOL_CONTRACTED = 5e-12
+
+
 def test_contracted_and_range_separated(gpu_ctx, oracle_mod):
     b = systems.water_system(1, 6.0, "gh", seed=5, contracted=True)
     o = oracle_mod.Oracle(b)
@@ -56,7 +67,7 @@ def test_contracted_and_range_separated(gpu_ctx, oracle_mod):
     for kappa in (0.0, 1.0 / 0.6 ** 2):
         for q, G in zip(quartets, gpu_ctx.eri_quartets(quartets, kappa=kappa)):
             R = o.eri(*q, kappa=kappa)
-            assert np.abs(G - R).max() <= TOL, (q, kappa)
+            assert np.abs(G - R).max() <= TOL_CONTRACTED, (q, kappa)
 
 
 def test_cholesky_with_g_and_h_shells(gpu_ctx, oracle_mod):

@@ -7,6 +7,7 @@ import pytest
 
 import pyoracle
 from gigpu import lib, systems
+from gigpu.basis import build_basis
 from grad_helpers import fd_gradient, random_density, tiny_basis
 
 pytestmark = pytest.mark.gpu
@@ -133,9 +134,16 @@ def test_autograd_function(ctx):
     assert abs((ep - em) / (2 * h) - float((R.grad * v).sum())) <= 1e-7
 
 
-def test_gradient_rejects_pairs_beyond_kernels(ctx):
-    basis = tiny_basis(lmax=3, contracted=True)   # an (f f) pair has l0 + l1 = 6
+def test_gradient_rejects_h_shells(ctx):
+    """The gradient's R tables reach order 4 lmax + 1, within the Boys table
+    (order 20) for l <= 4; a basis with an h shell gets the energy and a
+    clear error for the gradient."""
+    coords = np.array([[0.0, 0.1, -0.2], [1.4, 1.1, 0.3]])
+    spec = {"O": [(0, [0.45], [1.0]), (5, [0.9], [1.0])], "H": [(0, [0.3], [1.0])]}
+    basis = build_basis(["O", "H"], coords, spec)
     D = random_density(basis.nao, seed=9)
     ctx.set_basis(basis)
-    with pytest.raises(lib.GigpuError, match="l0 \\+ l1 <= 5"):
+    e = ctx.coulomb_energy(D, grad=False)
+    assert np.isfinite(e) and e > 0
+    with pytest.raises(lib.GigpuError, match="l <= 4"):
         ctx.coulomb_energy(D)
