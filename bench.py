@@ -290,6 +290,8 @@ def main():
                     help="skip the fixed-decode and raw-tie builds beside the headline build")
     ap.add_argument("--eri-stats", action="store_true",
                     help="print per-class ERI kernel statistics to stderr (sets GIGPU_ERI_STATS)")
+    ap.add_argument("--option", action="append", default=[], metavar="NAME=VALUE",
+                    help="launch-layer option of the headline context (gi_set_option), e.g. eri_md4=0")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="before the run, one device-to-device copy of this many MiB (a dispatch of known "
                          "size for scaling hardware byte counters)")
@@ -311,6 +313,9 @@ def main():
     Cmo, _ = np.linalg.qr(rng.normal(size=(basis.nao, basis.nao)))
     Cmo = np.asfortranarray(Cmo)
     ctx = lib.Context(0)
+    for opt in args.option:
+        name, value = opt.split("=", 1)
+        ctx.set_option(name, int(value))
     hip = Hip()
     if args.copy_probe_mb > 0:
         nbytes = args.copy_probe_mb << 20

@@ -11,6 +11,7 @@
 #include <unordered_map>
 #include "gi_context.h"
 #include "md_generic.h"
+#include "md4_kernel.h"
 #include "grad_kernels.h"
 #include "onee_kernels.h"
 
@@ -150,13 +151,21 @@ static void launch_lane(const MdArgs &a, int grid, hipStream_t s) {
   hipLaunchKernelGGL((md_lane_kernel<E, F>), dim3((unsigned)grid), dim3(256), 0, s, a);
 }
 
+using Launch4Fn = void (*)(const MdArgs &, const ColTile *, long long, int grid, hipStream_t);
+
+template <int E, int F>
+static void launch_md4(const MdArgs &a, const ColTile *ct, long long nct, int grid, hipStream_t s) {
+  hipLaunchKernelGGL((md4_kernel<E, F>), dim3((unsigned)grid), dim3(64), 0, s, a, ct, nct);
+}
+
 static LaunchFn md_table[MD_EMAX + 1][MD_EMAX + 1];
 static LaunchFn lane_table[MD_EMAX + 1][MD_EMAX + 1];   // null where a class has no lane kernel
+static Launch4Fn md4_table[MD_EMAX + 1][MD_EMAX + 1];
 
 static void init_md_table() {
   static bool done = false;
   if (done) return;
-#define GI_MD(e, f) md_table[e][f] = &launch_md<e, f>;
+#define GI_MD(e, f) md_table[e][f] = &launch_md<e, f>; md4_table[e][f] = &launch_md4<e, f>;
 #define GI_LANE(e, f) lane_table[e][f] = &launch_lane<e, f>;
 #include "md_classes.inc"
 #undef GI_MD
@@ -227,6 +236,43 @@ static int pair_rows_host(const PairDesc &d, int n0, int n1) {
   if (d.flags & PAIR_SEL) return d.nsel;
   if (d.flags & PAIR_SAME) return n0 * (n0 + 1) / 2;
   return n0 * n1;
+}
+
+// Column tiles of an md4 launch: the ket pairs' columns in groups of four,
+// four groups per tile, at most qmax distinct ket pairs per tile.  Returns the
+// number of R tables the tiles need (sum of their distinct ket pairs).
+static long long build_col_tiles(const std::vector<int> &ncol, int qmax, std::vector<ColTile> &out) {
+  ColTile cur{};
+  int nslot = 0;
+  long long nrt = 0;
+  auto reset = [&]() {
+    cur = ColTile{};
+    for (int s = 0; s < 4; ++s) {
+      cur.c0[s] = -1;
+      cur.jq[s] = 0;
+    }
+    cur.nq = 0;
+    nslot = 0;
+  };
+  auto flush = [&]() {
+    if (nslot == 0) return;
+    for (int q = cur.nq; q < 4; ++q) cur.jq[q] = cur.jq[0];
+    nrt += cur.nq;
+    out.push_back(cur);
+    reset();
+  };
+  reset();
+  for (int j = 0; j < (int)ncol.size(); ++j)
+    for (int c0 = 0; c0 < ncol[j]; c0 += 4) {
+      const bool newq = cur.nq == 0 || cur.jq[cur.nq - 1] != j;
+      if (nslot == 4 || (newq && cur.nq == qmax)) flush();
+      if (cur.nq == 0 || cur.jq[cur.nq - 1] != j) cur.jq[cur.nq++] = j;
+      cur.c0[nslot] = c0;
+      cur.rt[nslot] = cur.nq - 1;
+      ++nslot;
+    }
+  flush();
+  return nrt;
 }
 
 void eri_run(Context &C, const EriRequest &R) {
@@ -348,12 +394,14 @@ void eri_run(Context &C, const EriRequest &R) {
   struct Prepared {
     MdArgs a;
     long long grid;
-    int kind;   // 0 MFMA kernel, 1 lane kernel, 2 runtime-order kernel
+    int kind;   // 0 MFMA kernel, 1 lane kernel, 2 runtime-order kernel, 3 4x4x4 kernel
     int KE, KF, E, F;
     long long nq;
     double fl;
+    long long ct0, nct;   // kind 3: column tiles (offset into ctiles)
   };
   std::vector<Prepared> prep;
+  std::vector<ColTile> ctiles;
   auto nrow_of = [&](const PairDesc &d) {
     return (double)pair_rows_host(d, C.shell_nang[d.s0], C.shell_nang[d.s1]);
   };
@@ -386,6 +434,47 @@ void eri_run(Context &C, const EriRequest &R) {
       for (int r : L.rows) sr += nrow_of(rd[r]);
       for (int c : L.cols) sc += nrow_of(cd[c]);
       trans = sc / (double)ncols > sr / (double)nrows;
+    }
+    // 4x4x4 kernel (products): orientation and column tiles from its padded
+    // instruction count -- per tile and bra primitive pair, NHG (KSF ket
+    // primitives + ceil(nr / 4)) instructions, plus the R tables (about one
+    // instruction's time per 40 recursion entries)
+    bool use4 = !lane_path && !generic && R.mode == LAUNCH_PRODUCT && C.eri_md4;
+    std::vector<ColTile> t4[2];
+    if (use4) {
+      double best = 0;
+      int bo = 0;
+      for (int o = 0; o < 2; ++o) {
+        if (o == 1 && !C.eri_trans) break;
+        const int e = o ? L.F : L.E, f = o ? L.E : L.F;
+        const std::vector<PairDesc> &ord = o ? cd : rd, &ocd = o ? rd : cd;
+        const std::vector<int> &orows = o ? L.cols : L.rows, &ocols = o ? L.rows : L.cols;
+        std::vector<int> nc;
+        nc.reserve(ocols.size());
+        double np_c = 0;
+        for (int c : ocols) {
+          nc.push_back((int)nrow_of(ocd[c]));
+          np_c += nprim_of(ocd[c]);
+        }
+        np_c /= (double)std::max<size_t>(1, ocols.size());
+        const long long nrt = build_col_tiles(nc, m4_qmax(e + f), t4[o]);
+        const double nhg = (nherm(e) + 3) / 4, ksf = (nherm(f) + 3) / 4;
+        double rentries = 0;
+        for (int m = 0; m <= e + f; ++m) rentries += nherm(m);
+        double sum_rows = 0, sum_np = 0;
+        for (int r : orows) {
+          const double p = nprim_of(ord[r]);
+          sum_rows += p * std::ceil(nrow_of(ord[r]) / 4.0);
+          sum_np += p;
+        }
+        const double cost = (double)t4[o].size() * nhg * (ksf * np_c * sum_np + sum_rows) +
+                            (double)nrt * np_c * sum_np * rentries / 40.0;
+        if (o == 0 || cost < best) {
+          best = cost;
+          bo = o;
+        }
+      }
+      trans = bo == 1;
     }
     const int KE = trans ? L.F : L.E, KF = trans ? L.E : L.F;   // kernel bra / ket orders
     const std::vector<PairDesc> &krd = trans ? cd : rd, &kcd = trans ? rd : cd;
@@ -439,7 +528,16 @@ void eri_run(Context &C, const EriRequest &R) {
     a.sel = R.d_sel;
     a.boys = C.dboys.p;
     long long grid;
-    if (lane_path) {
+    long long ct0 = 0, nct = 0;
+    if (use4) {
+      std::vector<ColTile> &tl = t4[trans ? 1 : 0];
+      ct0 = (long long)ctiles.size();
+      nct = (long long)tl.size();
+      ctiles.insert(ctiles.end(), tl.begin(), tl.end());
+      a.ntiles = nrows * nct;
+      grid = std::min<long long>(a.ntiles, 8192);
+      if (grid >= 8) grid = (grid + 7) / 8 * 8;
+    } else if (lane_path) {
       const long long ng = (nrows + 63) / 64;
       const long long nt = (R.mode == LAUNCH_PRODUCT) ? ng * ncols : ng;
       a.ntiles = nt;
@@ -461,12 +559,14 @@ void eri_run(Context &C, const EriRequest &R) {
       grid = std::min<long long>(a.ntiles, 4096);
       if (grid >= 8) grid = (grid + 7) / 8 * 8;
     }
-    prep.push_back(Prepared{a, grid, generic ? 2 : (lane_path ? 1 : 0), KE, KF, L.E, L.F, nq, fl});
+    prep.push_back(Prepared{a, grid, use4 ? 3 : (generic ? 2 : (lane_path ? 1 : 0)), KE, KF, L.E, L.F, nq, fl,
+                            ct0, nct});
     nq_total += nq;
     flops_total += fl;
   }
   // per-class statistics time every launch on its own, so they stay on the
   // context stream; otherwise the launches are spread over the side streams
+  if (!ctiles.empty()) C.md_ctile.upload(ctiles, C.stream);
   const int nstream = stats ? 1 : (int)std::min<long long>(C.eri_streams, (long long)prep.size());
   if (nstream > 1) {
     if (!C.eri_fork) {
@@ -494,7 +594,9 @@ void eri_run(Context &C, const EriRequest &R) {
       GI_HIP(hipEventCreate(&s1));
       GI_HIP(hipEventRecord(s0, st));
     }
-    if (P.kind == 2)
+    if (P.kind == 3)
+      md4_table[P.KE][P.KF](P.a, C.md_ctile.p + P.ct0, P.nct, (int)P.grid, st);
+    else if (P.kind == 2)
       launch_generic(P.KE, P.KF, P.a, st);
     else
       (P.kind == 1 ? lane_table[P.KE][P.KF] : md_table[P.KE][P.KF])(P.a, (int)P.grid, st);

@@ -46,6 +46,7 @@ Context::~Context() {
   md_idx.release();
   md_side.release();
   md_rec.release();
+  md_ctile.release();
   chol.inv_l.release();
   chol.rkpq.release();
   chol.mo_adj_off.release();

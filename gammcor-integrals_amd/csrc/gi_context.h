@@ -9,6 +9,7 @@
 #include <memory>
 #include <vector>
 #include "md_kernel.h"
+#include "md4_kernel.h"
 #include "gi_common.h"
 
 namespace gi {
@@ -115,6 +116,7 @@ struct Context {
   DBuf<int> md_idx;
   DBuf<unsigned char> md_side;
   DBuf<QRec> md_rec;
+  DBuf<ColTile> md_ctile;   // column tiles of the request's 4x4x4 launches
   void reset_pair_cache() {
     md_pt.clear();
     md_hdr_host.clear();
@@ -139,6 +141,7 @@ struct Context {
   // launch-layer tuning (gi_set_option)
   long long eri_min_tiles = 8192, eri_max_kt = 32;
   bool eri_lane = true, eri_trans = true;
+  bool eri_md4 = true;   // products on the 4x4x4 kernel (md4_kernel.h) instead of md_kernel
   long long eri_lane_emax = LANE_EMAX;   // lane kernels only for classes with E, F <= this
   long long vec_blocks = 20;             // block rows of the triangular Inv_L Vabrs^T product
   // class launches of one ERI request are spread over this many streams (the

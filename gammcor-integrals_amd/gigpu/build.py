@@ -28,7 +28,7 @@ LIBS = ["-L/opt/rocm/lib", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"]
 
 SOURCES = ["eri.hip", "chol.hip", "mo.hip", "grad.hip", "split.hip", "onee.hip", "aomap.hip", "gi_tables.cpp",
            "gi_context.cpp", "sort.cpp", "capi.cpp"]
-HEADERS = ["gi_common.h", "gi_context.h", "md_kernel.h", "md_generic.h", "md_classes.inc", "grad_kernels.h", "onee_kernels.h"]
+HEADERS = ["gi_common.h", "gi_context.h", "md_kernel.h", "md4_kernel.h", "md_generic.h", "md_classes.inc", "grad_kernels.h", "onee_kernels.h"]
 
 
 def _compile(src: str, incremental: bool) -> str:

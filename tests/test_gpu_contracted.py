@@ -53,10 +53,12 @@ def test_contracted_eri_classes(gpu_ctx, oracle_mod, name, spher):
     print("%s: %d quartets, max |gpu - oracle| = %.2e" % (name, len(quartets), worst))
 
 
+@pytest.mark.parametrize("md4", [1, 0], ids=["md4", "md_kernel"])
 @pytest.mark.parametrize("name,spher", BASES, ids=[n for n, _ in BASES])
-def test_contracted_pair_block(gpu_ctx, oracle_mod, name, spher):
-    """The product launches (MFMA tiles with stacked kets, lane kernels,
-    transposed classes) on contracted shell pairs."""
+def test_contracted_pair_block(gpu_ctx, oracle_mod, name, spher, md4):
+    """The product launches (4x4x4 column tiles with ket and bra primitive
+    loops, 16x16x4 tiles with stacked kets, lane kernels, transposed
+    classes) on contracted shell pairs."""
     b = _basis(2, "large", 43, spher)
     o = oracle_mod.Oracle(b)
     gpu_ctx.set_basis(b)
@@ -65,9 +67,11 @@ def test_contracted_pair_block(gpu_ctx, oracle_mod, name, spher):
     rp, cp = pairs(31), pairs(29)
     try:
         gpu_ctx.set_option("eri_min_tiles", 1)
+        gpu_ctx.set_option("eri_md4", md4)
         V = gpu_ctx.eri_pair_block(rp, cp)
     finally:
         gpu_ctx.set_option("eri_min_tiles", 8192)
+        gpu_ctx.set_option("eri_md4", 1)
     nang = b.shell_nang
     r0 = 0
     for (A, B) in rp:

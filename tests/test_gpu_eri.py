@@ -85,12 +85,15 @@ THICK = [(3, 3), (3, 2), (2, 3), (2, 2), (3, 1)]
 
 @pytest.mark.parametrize("rows,cols", [(ALL_L, ALL_L), (THICK, THIN), (THIN, THICK)],
                          ids=["mixed", "thin-kets", "thin-bras"])
-@pytest.mark.parametrize("lane,trans,kt", [(1, 1, 32), (0, 1, 32), (0, 0, 32), (0, 1, 1)],
-                         ids=["default", "mfma-only", "no-transpose", "one-ket-tiles"])
-def test_pair_block_launch_shapes(setup, rows, cols, lane, trans, kt):
-    """The product launches with every kernel choice: ket stacking across
-    pairs of one tile, transposed orientation, MFMA kernels for the small
-    classes, single-ket tiles."""
+@pytest.mark.parametrize("lane,trans,kt,md4", [(1, 1, 32, 1), (0, 1, 32, 1), (0, 0, 32, 1), (0, 1, 32, 0),
+                                               (0, 0, 32, 0), (0, 1, 1, 0)],
+                         ids=["default", "md4-only", "md4-no-transpose", "md_kernel", "md_kernel-no-transpose",
+                              "md_kernel-one-ket-tiles"])
+def test_pair_block_launch_shapes(setup, rows, cols, lane, trans, kt, md4):
+    """The product launches with every kernel choice: the 4x4x4 kernel
+    (column tiles of four slots, per-class orientation), the 16x16x4 kernel
+    (ket stacking across pairs of one tile, transposed orientation,
+    single-ket tiles), lane kernels for the small classes."""
     b, o, ctx = setup
     rng = np.random.default_rng(7)
     rp, cp = _pairs(b, rng, rows, 37), _pairs(b, rng, cols, 41)
@@ -99,12 +102,14 @@ def test_pair_block_launch_shapes(setup, rows, cols, lane, trans, kt):
         ctx.set_option("eri_max_kt", kt)
         ctx.set_option("eri_lane", lane)
         ctx.set_option("eri_trans", trans)
+        ctx.set_option("eri_md4", md4)
         V = ctx.eri_pair_block(rp, cp)
     finally:
         ctx.set_option("eri_min_tiles", 8192)
         ctx.set_option("eri_max_kt", 32)
         ctx.set_option("eri_lane", 1)
         ctx.set_option("eri_trans", 1)
+        ctx.set_option("eri_md4", 1)
     nang = b.shell_nang
     r0 = 0
     worst = 0.0
