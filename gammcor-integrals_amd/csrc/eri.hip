@@ -151,11 +151,14 @@ static void launch_lane(const MdArgs &a, int grid, hipStream_t s) {
   hipLaunchKernelGGL((md_lane_kernel<E, F>), dim3((unsigned)grid), dim3(256), 0, s, a);
 }
 
-using Launch4Fn = void (*)(const MdArgs &, const ColTile *, long long, int grid, hipStream_t);
+using Launch4Fn = void (*)(const MdArgs &, const ColTile *, long long, bool plain, int grid, hipStream_t);
 
 template <int E, int F>
-static void launch_md4(const MdArgs &a, const ColTile *ct, long long nct, int grid, hipStream_t s) {
-  hipLaunchKernelGGL((md4_kernel<E, F>), dim3((unsigned)grid), dim3(64), 0, s, a, ct, nct);
+static void launch_md4(const MdArgs &a, const ColTile *ct, long long nct, bool plain, int grid, hipStream_t s) {
+  if (plain)
+    hipLaunchKernelGGL((md4_kernel<E, F, true>), dim3((unsigned)grid), dim3(64), 0, s, a, ct, nct);
+  else
+    hipLaunchKernelGGL((md4_kernel<E, F, false>), dim3((unsigned)grid), dim3(64), 0, s, a, ct, nct);
 }
 
 static LaunchFn md_table[MD_EMAX + 1][MD_EMAX + 1];
@@ -399,6 +402,7 @@ void eri_run(Context &C, const EriRequest &R) {
     long long nq;
     double fl;
     long long ct0, nct;   // kind 3: column tiles (offset into ctiles)
+    bool plain;           // kind 3: every pair of the launch uncontracted
   };
   std::vector<Prepared> prep;
   std::vector<ColTile> ctiles;
@@ -534,7 +538,7 @@ void eri_run(Context &C, const EriRequest &R) {
       ct0 = (long long)ctiles.size();
       nct = (long long)tl.size();
       ctiles.insert(ctiles.end(), tl.begin(), tl.end());
-      a.ntiles = nrows * nct;
+      a.ntiles = (nrows + M4_NB - 1) / M4_NB * nct;   // row blocks of M4_NB bra pairs x column tiles
       grid = std::min<long long>(a.ntiles, 8192);
       if (grid >= 8) grid = (grid + 7) / 8 * 8;
     } else if (lane_path) {
@@ -559,8 +563,11 @@ void eri_run(Context &C, const EriRequest &R) {
       grid = std::min<long long>(a.ntiles, 4096);
       if (grid >= 8) grid = (grid + 7) / 8 * 8;
     }
+    bool plain = true;
+    for (size_t k = 0; plain && k < L.rows.size(); ++k) plain = nprim_of(rd[L.rows[k]]) == 1;
+    for (size_t k = 0; plain && k < L.cols.size(); ++k) plain = nprim_of(cd[L.cols[k]]) == 1;
     prep.push_back(Prepared{a, grid, use4 ? 3 : (generic ? 2 : (lane_path ? 1 : 0)), KE, KF, L.E, L.F, nq, fl,
-                            ct0, nct});
+                            ct0, nct, plain});
     nq_total += nq;
     flops_total += fl;
   }
@@ -595,7 +602,7 @@ void eri_run(Context &C, const EriRequest &R) {
       GI_HIP(hipEventRecord(s0, st));
     }
     if (P.kind == 3)
-      md4_table[P.KE][P.KF](P.a, C.md_ctile.p + P.ct0, P.nct, (int)P.grid, st);
+      md4_table[P.KE][P.KF](P.a, C.md_ctile.p + P.ct0, P.nct, P.plain, (int)P.grid, st);
     else if (P.kind == 2)
       launch_generic(P.KE, P.KF, P.a, st);
     else

@@ -48,19 +48,72 @@ __device__ __forceinline__ int pick4(const int (&v)[4], int s) {
   return s == 0 ? v[0] : (s == 1 ? v[1] : (s == 2 ? v[2] : v[3]));
 }
 
+// 64-bit pointer made provably uniform (scalar registers), so that loads
+// through it with a 32-bit lane offset use the scalar-base addressing form
+__device__ __forceinline__ const double *uniform_ptr(const double *p) {
+  const unsigned long long v = (unsigned long long)p;
+  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
+  const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
+  return (const double *)(((unsigned long long)hi << 32) | lo);
+}
+typedef const __attribute__((address_space(1))) char gchar_t;
+typedef const __attribute__((address_space(1))) double gdouble_t;
+__device__ __forceinline__ double ld_at(const double *ubase, unsigned byte_off) {
+  return *(gdouble_t *)((gchar_t *)ubase + byte_off);
+}
+
+// bra pairs per tile (rows of one column tile handled by one wavefront)
+constexpr int M4_NB = 4;
+
 template <int E, int F>
+struct Md4 {
+  static constexpr int L = E + F, L1 = L + 1;
+  static constexpr int NHE = nherm(E), NHF = nherm(F), NHL = nherm(L);
+  static constexpr int NHG = (NHE + 3) / 4, KSF = (NHF + 3) / 4;
+  static constexpr int DS = m4_dsize(L), QM = m4_qmax(L), NL1 = L >= 1 ? nherm(L - 1) : 1;
+  static constexpr int NRMAX = nab_max(E);
+};
+
+// Boys function and top recursion level of one quartet (one lane): the
+// scaled F_m (2 pi^(5/2) / (p q sqrt(p + q + kappa p q)) (-2 alpha)^m F_m)
+// into F[0..L], P - Q into PQ[0..2]
+template <int L>
+__device__ __forceinline__ void md4_boys(const PrimRec &PB, const PrimRec &PK, double kappa,
+                                         const double *__restrict__ boys, double *F, double *PQ) {
+  const double PQx = PB.px - PK.px, PQy = PB.py - PK.py, PQz = PB.pz - PK.pz;
+  const double pq = PB.p * PK.p;
+  const double den = PB.p + PK.p + kappa * pq;
+  const double alpha = pq / den;
+  const double N = 34.986836655249725 / (pq * sqrt(den));   // 2 pi^(5/2)
+  double Fm[L + 1];
+  boys_fm<L>(alpha * (PQx * PQx + PQy * PQy + PQz * PQz), boys, Fm);
+  const double m2a = -2.0 * alpha;
+  double pw = N;
+#pragma unroll
+  for (int n = 0; n <= L; ++n) {
+    F[n] = Fm[n] * pw;
+    pw *= m2a;
+  }
+  PQ[0] = PQx;
+  PQ[1] = PQy;
+  PQ[2] = PQz;
+}
+
+// PLAIN: every pair of the launch has one primitive pair (uncontracted), so the
+// kernel omits the primitive loops
+template <int E, int F, bool PLAIN>
 __global__ void __launch_bounds__(64) MD4_ATTR md4_kernel(MdArgs A, const ColTile *__restrict__ ctiles, long long nct) {
-  constexpr int L = E + F, L1 = L + 1;
-  constexpr int NHE = nherm(E), NHF = nherm(F), NHL = nherm(L);
-  constexpr int NHG = (NHE + 3) / 4, KSF = (NHF + 3) / 4;
-  constexpr int DS = m4_dsize(L), QM = m4_qmax(L), NL1 = L >= 1 ? nherm(L - 1) : 1;
-  __shared__ double s_Rd[QM][DS];        // level-0 R tables, dense
-  __shared__ double s_Rp[2][QM][NL1];    // levels >= 1, packed (ping-pong)
-  __shared__ double s_F[QM][L + 1];      // scaled Boys values per quartet
-  __shared__ double s_PQ[QM][4];         // P - Q, active flag
+  using S = Md4<E, F>;
+  constexpr int L = S::L, L1 = S::L1, NHE = S::NHE, NHF = S::NHF, NHL = S::NHL, NHG = S::NHG, KSF = S::KSF;
+  constexpr int DS = S::DS, QM = S::QM, NL1 = S::NL1;
+  __shared__ double s_Rd[QM][DS];              // level-0 R tables, dense
+  __shared__ double s_Rp[2][QM][NL1];          // levels >= 1, packed (ping-pong)
+  __shared__ double s_F[M4_NB][QM][L + 1];     // scaled Boys values per (bra pair, quartet)
+  __shared__ double s_PQ[M4_NB][QM][4];        // P - Q per (bra pair, quartet)
   __shared__ int s_rrec[NHL];
-  __shared__ unsigned short s_dix[NHL];  // dense offset of packed index
-  __shared__ int s_hoff[NHG * 4];        // dense offset of h (0 for padding)
+  __shared__ unsigned short s_dix[NHL];        // dense offset of packed index
+  __shared__ int s_hoff[NHG * 4];              // dense offset of h (0 for padding)
+  __shared__ unsigned s_frow[S::NRMAX + 4];    // byte offset of each bra row's full-block index
 
   const int lane = threadIdx.x;
   const int lq = lane >> 4, lb = (lane >> 2) & 3, li = lane & 3;
@@ -96,19 +149,60 @@ __global__ void __launch_bounds__(64) MD4_ATTR md4_kernel(MdArgs A, const ColTil
   int hoff[NHG];   // dense offset of this lane's h = 4 hg + li
 #pragma unroll
   for (int hg = 0; hg < NHG; ++hg) hoff[hg] = s_hoff[4 * hg + li];
+  // the last h step pads h beyond NHE (rows h = 4 hg + lq of the stage-2 A operand)
+  const bool hlast_ok = 4 * (NHG - 1) + lq < NHE;
+  constexpr bool hg_last_mask = (NHE % 4) != 0;
 
-  // XCD-aware order: consecutive tiles (one bra pair) on one XCD's L2
+  // XCD-aware order: consecutive tiles (one row block) on one XCD's L2
   const int G = gridDim.x;
   const int w = blockIdx.x;
   const long long v0 = (G % 8 == 0) ? (long long)(w % 8) * (G / 8) + w / 8 : w;
   const long long rs = A.trans ? A.ldo : 1, cs = A.trans ? 1 : A.ldo;
+  const long long nrb = (A.nrows + M4_NB - 1) / M4_NB;
+  const long long ntiles = nrb * nct;
 
-  for (long long t = v0; t < A.ntiles; t += G) {
-    const long long ir = t / nct, ct = t - ir * nct;
-    const QRec RB = A.rrec[ir];
+  // R tables of the nq quartets of bra pair b (Boys values in s_F[b], s_PQ[b]):
+  // the recursion R^(n)_{tuv} = coef R^(n+1)_{i2} + PQ_dir R^(n+1)_{i1},
+  // levels L-1 .. 0, lpq lanes per quartet; level 0 into the dense tables
+  auto build_r = [&](int b, int nq, unsigned actmask) {
+    const int lpq = nq <= 1 ? 64 : (nq == 2 ? 32 : 16);
+    const int ql = lane / lpq, sub = lane - ql * lpq;
+    const bool act = ql < nq && ((actmask >> ql) & 1u);
+    __syncthreads();   // Boys values written; earlier readers of the tables are done
+    if (act && L > 0 && sub == 0) s_Rp[L % 2][ql][0] = s_F[b][ql][L];
+    if (act && L == 0 && sub == 0) s_Rd[ql][0] = s_F[b][ql][0];
+#pragma unroll
+    for (int n = L - 1; n >= 0; --n) {
+      __syncthreads();
+      const int cnt = nherm(L - n);
+      if (act) {
+        const double *prv = s_Rp[(n + 1) % 2][ql];
+        for (int idx = sub; idx < cnt; idx += lpq) {
+          double r;
+          if (idx == 0) {
+            r = s_F[b][ql][n];
+          } else {
+            const int rec = s_rrec[idx];
+            const int i1 = rec & 0x7ff, i2 = (rec >> 11) & 0x7ff, dir = (rec >> 22) & 3, coef = rec >> 24;
+            r = (double)coef * prv[i2] + s_PQ[b][ql][dir] * prv[i1];
+          }
+          if (n == 0)
+            s_Rd[ql][s_dix[idx]] = r;
+          else
+            s_Rp[n % 2][ql][idx] = r;
+        }
+      }
+    }
+    __syncthreads();
+  };
+
+  for (long long t = v0; t < ntiles; t += G) {
+    const long long rb = t / nct, ct = t - rb * nct;
+    const long long ir0 = rb * M4_NB;
+    const int nb = (int)((A.nrows - ir0) < M4_NB ? (A.nrows - ir0) : M4_NB);
     const ColTile T = ctiles[ct];
     const int nq = T.nq;
-    // this lane's slot: ket pair, column
+    // this lane's slot: ket pair, column (shared by the tile's bra pairs)
     const int myrt = pick4(T.rt, lb);
     const int myc0 = pick4(T.c0, lb);
     const QRec CD = A.crec[pick4(T.jq, myrt)];
@@ -116,133 +210,140 @@ __global__ void __launch_bounds__(64) MD4_ATTR md4_kernel(MdArgs A, const ColTil
     const bool cval = myc0 >= 0 && c < CD.nrow;
     const int fc = cval ? pair_full(CD, A.sel, c) : 0;
     const int ldk = CD.n0 * CD.n1;
-    int maxnp = 1;
-    for (int qq = 0; qq < nq; ++qq) maxnp = imax(maxnp, A.crec[pick4(T.jq, qq)].nprim);
-    const int nr = RB.nrow, ldb = RB.n0 * RB.n1;
-    double *const out = A.out + RB.off + CD.off;
-    const int nrg = (nr + 3) / 4;
 
-    for (int bp = 0; bp < RB.nprim; ++bp) {
-      const PrimRec PB = bp == 0 ? first_prim(RB) : A.prim[RB.prim0 + bp];
-      const double *etb = A.et + PB.et;
-      double X[NHG];
+    // stage-1 B operand of ket primitive kp: (-1)^{|k|} ET_ket(k, c)
+    double eb[KSF];
+    auto load_eb = [&](int kp) {
+      const bool kval = cval && kp < CD.nprim;
+      const int kpc = kp < CD.nprim ? kp : CD.nprim - 1;
+      const PrimRec PKm = kpc == 0 ? first_prim(CD) : A.prim[CD.prim0 + kpc];
+      const double *etk = A.et + PKm.et;
 #pragma unroll
-      for (int hg = 0; hg < NHG; ++hg) X[hg] = 0.0;
-
-      for (int kp = 0; kp < maxnp; ++kp) {
-        __syncthreads();   // earlier readers of the R tables are done
-        // Boys function and top level of each distinct quartet
-        if (lane < nq) {
-          const QRec CQ = A.crec[pick4(T.jq, lane)];
-          const bool act = kp < CQ.nprim;
-          double PQx = 0.0, PQy = 0.0, PQz = 0.0;
-          if (act) {
-            const PrimRec PK = kp == 0 ? first_prim(CQ) : A.prim[CQ.prim0 + kp];
-            PQx = PB.px - PK.px;
-            PQy = PB.py - PK.py;
-            PQz = PB.pz - PK.pz;
-            const double pq = PB.p * PK.p;
-            const double den = PB.p + PK.p + A.kappa * pq;
-            const double alpha = pq / den;
-            const double N = 34.986836655249725 / (pq * sqrt(den));   // 2 pi^(5/2)
-            double Fm[L + 1];
-            boys_fm<L>(alpha * (PQx * PQx + PQy * PQy + PQz * PQz), A.boys, Fm);
-            const double m2a = -2.0 * alpha;
-            double pw = N;
-#pragma unroll
-            for (int n = 0; n <= L; ++n) {
-              s_F[lane][n] = Fm[n] * pw;
-              if (n == L) {
-                if (L == 0)
-                  s_Rd[lane][0] = Fm[n] * pw;
-                else
-                  s_Rp[L % 2][lane][0] = Fm[n] * pw;
-              }
-              pw *= m2a;
-            }
-          }
-          s_PQ[lane][0] = PQx;
-          s_PQ[lane][1] = PQy;
-          s_PQ[lane][2] = PQz;
-          s_PQ[lane][3] = act ? 1.0 : 0.0;
-        }
-        // recursion R^(n)_{tuv} = coef R^(n+1)_{i2} + PQ_dir R^(n+1)_{i1}, levels L-1 .. 0
-#pragma unroll
-        for (int n = L - 1; n >= 0; --n) {
-          __syncthreads();
-          const int cnt = nherm(L - n);
-          const int tot = nq * cnt;
-          for (int e = lane; e < tot; e += 64) {
-            const int ql = e / cnt, idx = e - ql * cnt;
-            if (s_PQ[ql][3] == 0.0) continue;
-            double r;
-            if (idx == 0) {
-              r = s_F[ql][n];
-            } else {
-              const int rec = s_rrec[idx];
-              const int i1 = rec & 0x7ff, i2 = (rec >> 11) & 0x7ff, dir = (rec >> 22) & 3, coef = rec >> 24;
-              const double *prv = s_Rp[(n + 1) % 2][ql];
-              r = (double)coef * prv[i2] + s_PQ[ql][dir] * prv[i1];
-            }
-            if (n == 0)
-              s_Rd[ql][s_dix[idx]] = r;
-            else
-              s_Rp[n % 2][ql][idx] = r;
-          }
-        }
-        // stage-1 B operand: (-1)^{|k|} ET_ket(k, c) of the slot's ket primitive
-        const bool kval = cval && kp < CD.nprim;
-        const int kpc = kp < CD.nprim ? kp : CD.nprim - 1;
-        const PrimRec PKm = kpc == 0 ? first_prim(CD) : A.prim[CD.prim0 + kpc];
-        const double *etk = A.et + PKm.et;
-        // loads from clamped addresses, masked afterwards (no branches)
-        double eb[KSF];
-#pragma unroll
-        for (int kg = 0; kg < KSF; ++kg) {
-          const int k = 4 * kg + lq;
-          const bool ok = kval && k < NHF;
-          const double v = etk[ok ? (long long)k * ldk + fc : 0];
-          eb[kg] = ok ? (((ksgn >> kg) & 1u) ? -v : v) : 0.0;
-        }
-        __syncthreads();   // R tables complete
-        // k step kg: NHG independent accumulators (a scheduling barrier per
-        // step keeps the compiler from hoisting every read of the unrolled loop)
-        // The step's offset passes through an empty asm statement so that its
-        // NHG read addresses are formed in the step (one add each) instead of
-        // all NHG x KSF of them being hoisted into registers.
-        const double *Rq = s_Rd[myrt];
-#pragma unroll
-        for (int kg = 0; kg < KSF; ++kg) {
-          int ko = koff[kg];
-          asm volatile("" : "+v"(ko));
-          const double *Rk = Rq + ko;
-#pragma unroll
-          for (int hg = 0; hg < NHG; ++hg) X[hg] = __builtin_amdgcn_mfma_f64_4x4x4f64(Rk[hoff[hg]], eb[kg], X[hg], 0, 0, 0);
-          __builtin_amdgcn_sched_barrier(0);
-        }
+      for (int kg = 0; kg < KSF; ++kg) {
+        const int k = 4 * kg + lq;
+        const bool ok = kval && k < NHF;
+        const double v = etk[ok ? (long long)k * ldk + fc : 0];
+        eb[kg] = ok ? (((ksgn >> kg) & 1u) ? -v : v) : 0.0;
       }
-      // stage 2, one row group of four bra rows at a time (the A operand's
-      // row 4 rg + li): V(r, c) = sum_h ET_bra(h, r) X(h, c), two independent
-      // accumulator chains (even and odd h steps); lane (q, b, i) then holds
-      // V(4 rg + q, c0_b + i).  A contracted bra adds its later primitive
-      // pairs to the stored block.
+    };
+    // stage 1: X(h, c) += sum_k R_{h+k} B(k, c) from the dense table of this
+    // lane's slot; k step kg updates NHG independent accumulators (the step's
+    // offset passes through an empty asm statement so that the NHG addresses
+    // are formed in the step instead of all NHG x KSF hoisted into registers)
+    double X[NHG];
+    auto stage1 = [&]() {
+      const double *Rq = s_Rd[myrt];
+#pragma unroll
+      for (int kg = 0; kg < KSF; ++kg) {
+        int ko = koff[kg];
+        asm volatile("" : "+v"(ko));
+        const double *Rk = Rq + ko;
+#pragma unroll
+        for (int hg = 0; hg < NHG; ++hg) X[hg] = __builtin_amdgcn_mfma_f64_4x4x4f64(Rk[hoff[hg]], eb[kg], X[hg], 0, 0, 0);
+        __builtin_amdgcn_sched_barrier(0);
+      }
+    };
+    // bra rows of RB: byte offset of each row's full-block index
+    auto fill_rows = [&](const QRec &RB) {
+      for (int r = lane; r < RB.nrow; r += 64) s_frow[r] = 8u * (unsigned)pair_full(RB, A.sel, r);
+      if (lane < 4) s_frow[RB.nrow + lane] = 0u;   // padding rows of the last row group
+      __syncthreads();
+    };
+    // stage 2 for one bra primitive pair: V(r, c) = sum_h ET_bra(h, r) X(h, c),
+    // four bra rows per step (the A operand's row 4 rg + li); the table of h step
+    // hg starts at a uniform address, each lane reads at a 32-bit offset; two
+    // accumulator chains (even and odd h steps)
+    auto stage2 = [&](const QRec &RB, const double *etb, bool accumulate) {
+      const int nr = RB.nrow, ldb = RB.n0 * RB.n1;
+      const double *ub = uniform_ptr(etb);
+      const unsigned hstep = 32u * (unsigned)ldb;   // 4 h rows, bytes
+      const unsigned lrow = 8u * (unsigned)(lq * ldb);
+      // the last h step reads row min(h, NHE - 1) (inside the block), masked
+      const unsigned lrow_last = 8u * (unsigned)((imin(4 * (NHG - 1) + lq, NHE - 1) - 4 * (NHG - 1)) * ldb);
+      double *const out = A.out + RB.off + CD.off;
+      const int nrg = (nr + 3) / 4;
+      // the operands of row group rg + 1 are read while rg multiplies
+      double ea[NHG];
+      auto load_rows = [&](int rg) {
+        const unsigned fro = s_frow[4 * rg + li];
+#pragma unroll
+        for (int hg = 0; hg < NHG; ++hg) {
+          if (hg == NHG - 1 && (NHE % 4) != 0)
+            ea[hg] = ld_at(ub, lrow_last + fro + (unsigned)hg * hstep);
+          else
+            ea[hg] = ld_at(ub, lrow + fro + (unsigned)hg * hstep);
+        }
+      };
+      load_rows(0);
       for (int rg = 0; rg < nrg; ++rg) {
-        const int r = 4 * rg + li;
-        const int fr = pair_full(RB, A.sel, r < nr ? r : nr - 1);
+        double e[NHG];
+#pragma unroll
+        for (int hg = 0; hg < NHG; ++hg) e[hg] = ea[hg];
+        if (hg_last_mask) e[NHG - 1] = hlast_ok ? e[NHG - 1] : 0.0;
+        if (rg + 1 < nrg) load_rows(rg + 1);
         double v0 = 0.0, v1 = 0.0;
 #pragma unroll
         for (int hg = 0; hg < NHG; ++hg) {
-          const int h = 4 * hg + lq;
-          const double e = etb[(h < NHE ? h : NHE - 1) * ldb + fr];
           if (hg & 1)
-            v1 = __builtin_amdgcn_mfma_f64_4x4x4f64(h < NHE ? e : 0.0, X[hg], v1, 0, 0, 0);
+            v1 = __builtin_amdgcn_mfma_f64_4x4x4f64(e[hg], X[hg], v1, 0, 0, 0);
           else
-            v0 = __builtin_amdgcn_mfma_f64_4x4x4f64(h < NHE ? e : 0.0, X[hg], v0, 0, 0, 0);
+            v0 = __builtin_amdgcn_mfma_f64_4x4x4f64(e[hg], X[hg], v0, 0, 0, 0);
         }
         const int ro = 4 * rg + lq;
         if (cval && ro < nr) {
           double *d = out + rs * ro + cs * c;
-          *d = bp > 0 ? *d + (v0 + v1) : v0 + v1;
+          *d = accumulate ? *d + (v0 + v1) : v0 + v1;
+        }
+      }
+    };
+
+    if constexpr (PLAIN) {
+      // uncontracted tile: one Boys pass for every (bra pair, ket pair)
+      // quartet of the tile, then per bra pair the R tables and both stages
+      load_eb(0);
+      __syncthreads();   // earlier readers of s_F / s_PQ are done
+      if (lane < nb * nq) {
+        const int b = lane / nq, q = lane - b * nq;
+        const QRec RB = A.rrec[ir0 + b];
+        const QRec CQ = A.crec[pick4(T.jq, q)];
+        md4_boys<L>(first_prim(RB), first_prim(CQ), A.kappa, A.boys, s_F[b][q], s_PQ[b][q]);
+      }
+      for (int b = 0; b < nb; ++b) {
+        const QRec RB = A.rrec[ir0 + b];
+        fill_rows(RB);
+        build_r(b, nq, 0xfu);
+#pragma unroll
+        for (int hg = 0; hg < NHG; ++hg) X[hg] = 0.0;
+        stage1();
+        stage2(RB, A.et + RB.et, false);
+      }
+    } else {
+      // contracted pairs: per bra pair and bra primitive pair, X summed over
+      // the ket primitive pairs; the block accumulates over bra primitives
+      int maxnp = 1;
+      for (int qq = 0; qq < nq; ++qq) maxnp = imax(maxnp, A.crec[pick4(T.jq, qq)].nprim);
+      for (int b = 0; b < nb; ++b) {
+        const QRec RB = A.rrec[ir0 + b];
+        fill_rows(RB);
+        for (int bp = 0; bp < RB.nprim; ++bp) {
+          const PrimRec PB = bp == 0 ? first_prim(RB) : A.prim[RB.prim0 + bp];
+#pragma unroll
+          for (int hg = 0; hg < NHG; ++hg) X[hg] = 0.0;
+          for (int kp = 0; kp < maxnp; ++kp) {
+            __syncthreads();   // earlier readers of the R tables and s_F are done
+            unsigned act = 0;
+            for (int qq = 0; qq < nq; ++qq)
+              if (kp < A.crec[pick4(T.jq, qq)].nprim) act |= 1u << qq;
+            if (lane < nq && ((act >> lane) & 1u)) {
+              const QRec CQ = A.crec[pick4(T.jq, lane)];
+              const PrimRec PK = kp == 0 ? first_prim(CQ) : A.prim[CQ.prim0 + kp];
+              md4_boys<L>(PB, PK, A.kappa, A.boys, s_F[0][lane], s_PQ[0][lane]);
+            }
+            build_r(0, nq, act);
+            load_eb(kp);
+            stage1();
+          }
+          stage2(RB, A.et + PB.et, bp > 0);
         }
       }
     }
