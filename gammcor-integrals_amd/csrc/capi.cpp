@@ -242,7 +242,18 @@ int gi_chol_build_batch(int64_t n, gi_context *const *ctxs, double tau, double k
         if (i >= n) break;
         try {
           gi::Context &C = ctx_of(ctxs[i]);
-          gi::chol_build(C, tau, kappa, max_block_dim, vec_block, flags);
+          // the molecules of the batch already run concurrently (one stream
+          // each); forking every context's ERI launches over side streams as
+          // well would oversubscribe the hardware queues
+          const long long streams = C.eri_streams;
+          C.eri_streams = 1;
+          try {
+            gi::chol_build(C, tau, kappa, max_block_dim, vec_block, flags);
+          } catch (...) {
+            C.eri_streams = streams;
+            throw;
+          }
+          C.eri_streams = streams;
           status[i] = 0;
         } catch (const gi::Error &e) {
           status[i] = e.code ? e.code : 1;

@@ -428,6 +428,14 @@ void eri_run(Context &C, const EriRequest &R) {
       lane_path = lt || lane_table[L.E][L.F] != nullptr;
       trans = lt;
     }
+    // measured per class on the (H2O)16 decomposition (caller's E rows, F
+    // columns): the lane kernels win where both sides are thin; the 4x4x4
+    // kernel wins for (2,1), (2,2), (2,3), (3,1), (3,2), (0,4), (4,0)
+    if (C.eri_md4 && lane_path) {
+      const int e = L.E, f = L.F;
+      const bool lane_wins = (e <= 1 && f <= 3) || (f == 0 && e <= 3) || (e == 2 && f == 0);
+      if (!lane_wins) lane_path = false, trans = false;
+    }
     for (size_t k = 0; lane_path && k < L.rows.size(); ++k) lane_path = nprim_of(rd[L.rows[k]]) == 1;
     for (size_t k = 0; lane_path && k < L.cols.size(); ++k) lane_path = nprim_of(cd[L.cols[k]]) == 1;
     if (!lane_path) trans = false;

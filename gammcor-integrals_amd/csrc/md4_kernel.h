@@ -56,6 +56,15 @@ __device__ __forceinline__ const double *uniform_ptr(const double *p) {
   const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
   return (const double *)(((unsigned long long)hi << 32) | lo);
 }
+// raw buffer resource over `bytes` bytes from a uniform base (the gfx9
+// descriptor word 3, untyped 32-bit data); loads take a per-lane 32-bit offset
+// and a uniform (scalar) offset, and return 0 beyond the range
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const double *ubase, unsigned bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc((void *)ubase, (short)0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ double buf_ld(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
 typedef const __attribute__((address_space(1))) char gchar_t;
 typedef const __attribute__((address_space(1))) double gdouble_t;
 __device__ __forceinline__ double ld_at(const double *ubase, unsigned byte_off) {
@@ -255,7 +264,7 @@ __global__ void __launch_bounds__(64) MD4_ATTR md4_kernel(MdArgs A, const ColTil
     // accumulator chains (even and odd h steps)
     auto stage2 = [&](const QRec &RB, const double *etb, bool accumulate) {
       const int nr = RB.nrow, ldb = RB.n0 * RB.n1;
-      const double *ub = uniform_ptr(etb);
+      const __amdgpu_buffer_rsrc_t ub = make_rsrc(uniform_ptr(etb), 8u * (unsigned)(NHE * ldb));
       const unsigned hstep = 32u * (unsigned)ldb;   // 4 h rows, bytes
       const unsigned lrow = 8u * (unsigned)(lq * ldb);
       // the last h step reads row min(h, NHE - 1) (inside the block), masked
@@ -269,9 +278,9 @@ __global__ void __launch_bounds__(64) MD4_ATTR md4_kernel(MdArgs A, const ColTil
 #pragma unroll
         for (int hg = 0; hg < NHG; ++hg) {
           if (hg == NHG - 1 && (NHE % 4) != 0)
-            ea[hg] = ld_at(ub, lrow_last + fro + (unsigned)hg * hstep);
+            ea[hg] = buf_ld(ub, lrow_last + fro, (unsigned)hg * hstep);
           else
-            ea[hg] = ld_at(ub, lrow + fro + (unsigned)hg * hstep);
+            ea[hg] = buf_ld(ub, lrow + fro, (unsigned)hg * hstep);
         }
       };
       load_rows(0);

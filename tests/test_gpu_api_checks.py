@@ -56,9 +56,11 @@ def test_debug_accuracy_residual_below_tau(gpu_ctx, oracle_mod, flags):
     rmax, rmin = gpu_ctx.chol_residual()
     assert rmax <= tau * (1 + 1e-6)
     # (pq|pq) - sum_k R_k(pq)^2 >= 0 in exact arithmetic; the vectors pass
-    # through Inv_L, whose condition grows as tau falls, and the recomputed
-    # residual carries rounding errors of a few 1e-12 here
-    assert rmin >= -0.1 * tau
+    # through Inv_L, whose condition grows as tau falls (about 1e4 here), and
+    # the recomputed residual carries rounding errors of about 1e-11 (measured
+    # -1.2e-11).  Wrong vectors would leave residuals of the size of the
+    # diagonal (about 1), so half of tau still separates round-off from a defect.
+    assert rmin >= -0.5 * tau
 
 
 def test_mo_transform_passes_equal_one_pass(gpu_ctx):
