@@ -35,9 +35,12 @@ struct ColTile {
 };
 
 __host__ __device__ constexpr int m4_dsize(int L) { return (L + 1) * (L + 1) * (L + 1); }
-// distinct quartets per tile: their dense R tables within ~12 KB of LDS
+// distinct quartets per tile: their dense R tables within MD4_RBUDGET bytes of LDS
+#ifndef MD4_RBUDGET
+#define MD4_RBUDGET 9216
+#endif
 __host__ __device__ constexpr int m4_qmax(int L) {
-  return imax(1, imin(4, 12288 / (8 * m4_dsize(L))));
+  return imax(1, imin(4, MD4_RBUDGET / (8 * m4_dsize(L))));
 }
 
 #ifndef MD4_ATTR
@@ -65,14 +68,12 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const double *ubase,
 __device__ __forceinline__ double buf_ld(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
   return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
 }
-typedef const __attribute__((address_space(1))) char gchar_t;
-typedef const __attribute__((address_space(1))) double gdouble_t;
-__device__ __forceinline__ double ld_at(const double *ubase, unsigned byte_off) {
-  return *(gdouble_t *)((gchar_t *)ubase + byte_off);
-}
 
 // bra pairs per tile (rows of one column tile handled by one wavefront)
-constexpr int M4_NB = 4;
+#ifndef MD4_NB
+#define MD4_NB 4
+#endif
+constexpr int M4_NB = MD4_NB;
 
 template <int E, int F>
 struct Md4 {
@@ -277,10 +278,8 @@ __global__ void __launch_bounds__(64) MD4_ATTR md4_kernel(MdArgs A, const ColTil
         const unsigned fro = s_frow[4 * rg + li];
 #pragma unroll
         for (int hg = 0; hg < NHG; ++hg) {
-          if (hg == NHG - 1 && (NHE % 4) != 0)
-            ea[hg] = buf_ld(ub, lrow_last + fro, (unsigned)hg * hstep);
-          else
-            ea[hg] = buf_ld(ub, lrow + fro, (unsigned)hg * hstep);
+          const unsigned o = (hg == NHG - 1 && (NHE % 4) != 0) ? lrow_last + fro : lrow + fro;
+          ea[hg] = buf_ld(ub, o, (unsigned)hg * hstep);
         }
       };
       load_rows(0);
