@@ -405,8 +405,8 @@ def main():
                                         "potrf_trtri", "vectors", "vabrs", "vectors_gemm", "eri_kernel")},
         "rtr_tflops": tm["rtr_flops"] / tm["rtr"] / 1e12 if tm["rtr"] > 0 else None,
         "roofline": {"bound": "mfma",
-                     "kernel": "ERI kernel family: md_kernel (FP64 MFMA) + md_lane_kernel (FP64 VALU) + k_build_et; "
-                               "MI355X FP64 vector peak = FP64 matrix peak",
+                     "kernel": "ERI kernel family: md4_kernel (FP64 4x4x4 MFMA) + md_kernel (FP64 16x16x4 MFMA) + "
+                               "md_lane_kernel (FP64 VALU) + k_build_et; MI355X FP64 vector peak = FP64 matrix peak",
                      "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": (achieved / FP64_PEAK_TFLOPS) if achieved else None,
                      "traffic": profiled_traffic(),
