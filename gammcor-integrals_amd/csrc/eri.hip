@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstring>
 #include <map>
 #include <unordered_map>
 #include "gi_context.h"
@@ -160,6 +161,18 @@ static void launch_md4(const MdArgs &a, const ColTile *ct, long long nct, bool p
   else
     hipLaunchKernelGGL((md4_kernel<E, F, false>), dim3((unsigned)grid), dim3(64), 0, s, a, ct, nct);
 }
+
+using Launch4sFn = void (*)(const MdArgs &, bool plain, int grid, hipStream_t);
+
+template <int E>
+static void launch_md4s(const MdArgs &a, bool plain, int grid, hipStream_t s) {
+  if (plain)
+    hipLaunchKernelGGL((md4s_kernel<E, true>), dim3((unsigned)grid), dim3(64), 0, s, a);
+  else
+    hipLaunchKernelGGL((md4s_kernel<E, false>), dim3((unsigned)grid), dim3(64), 0, s, a);
+}
+static const Launch4sFn md4s_table[MD_EMAX + 1] = {&launch_md4s<0>, &launch_md4s<1>, &launch_md4s<2>, &launch_md4s<3>,
+                                                   &launch_md4s<4>, &launch_md4s<5>, &launch_md4s<6>};
 
 static LaunchFn md_table[MD_EMAX + 1][MD_EMAX + 1];
 static LaunchFn lane_table[MD_EMAX + 1][MD_EMAX + 1];   // null where a class has no lane kernel
@@ -326,21 +339,13 @@ void eri_run(Context &C, const EriRequest &R) {
     GI_HIP(hipEventCreate(&e1));
   }
   long long nlaunch = 0;
+  // device tables that outlive the request (pair headers, primitive records,
+  // Hermite tables) grow first: the launch arguments below point into them
   if (C.md_hdr_host.size() != nhdr0) C.md_hdr.upload(C.md_hdr_host, C.stream);
   if (!jobs.empty()) {
     C.md_prim.grow_keep((size_t)C.md_prim_used, C.stream);
     C.md_et.grow_keep((size_t)C.md_et_used, C.stream);
-    C.md_jobs.upload(jobs, C.stream);
-    if (C.time_eri) GI_HIP(hipEventRecord(e0, C.stream));
-    hipLaunchKernelGGL(k_build_et, dim3((unsigned)jobs.size()), dim3(64), 0, C.stream, C.dshells.p, C.dexps.p,
-                       C.dcntr.p, C.dnorms.p, C.spher, C.md_jobs.p, C.md_prim.p, C.md_et.p);
-    GI_HIP(hipGetLastError());
-    ++nlaunch;
-  } else if (C.time_eri) {
-    GI_HIP(hipEventRecord(e0, C.stream));
   }
-  C.md_rdesc.upload(rd, C.stream);
-  C.md_cdesc.upload(cd, C.stream);
 
   // ---- bins by E (rows) and F (columns)
   auto E_of = [&](const PairDesc &d) { return C.shell_l[d.s0] + C.shell_l[d.s1]; };
@@ -379,15 +384,7 @@ void eri_run(Context &C, const EriRequest &R) {
   std::vector<unsigned char> side(idx.size(), 0);
   for (size_t i = 0; i < launches.size(); ++i)
     std::fill(side.begin() + coff[i], side.begin() + coff[i] + launches[i].cols.size(), (unsigned char)1);
-  C.md_idx.upload(idx, C.stream);
-  C.md_side.upload(side, C.stream);
   C.md_rec.alloc(idx.size());
-  if (!idx.empty()) {
-    hipLaunchKernelGGL(k_pack_recs, dim3((unsigned)((idx.size() + 255) / 256)), dim3(256), 0, C.stream, C.md_rdesc.p,
-                       C.md_cdesc.p, C.md_idx.p, C.md_side.p, (long long)idx.size(), C.md_hdr.p, C.md_prim.p,
-                       C.md_rec.p);
-    GI_HIP(hipGetLastError());   // record packing: not counted among the ERI kernel launches
-  }
 
   static const bool stats = getenv("GIGPU_ERI_STATS") != nullptr;
   long long nq_total = 0;
@@ -397,7 +394,7 @@ void eri_run(Context &C, const EriRequest &R) {
   struct Prepared {
     MdArgs a;
     long long grid;
-    int kind;   // 0 MFMA kernel, 1 lane kernel, 2 runtime-order kernel, 3 4x4x4 kernel
+    int kind;   // 0 MFMA kernel, 1 lane kernel, 2 runtime-order kernel, 3 4x4x4 kernel, 4 its thin-ket form
     int KE, KF, E, F;
     long long nq;
     double fl;
@@ -439,9 +436,22 @@ void eri_run(Context &C, const EriRequest &R) {
     for (size_t k = 0; lane_path && k < L.rows.size(); ++k) lane_path = nprim_of(rd[L.rows[k]]) == 1;
     for (size_t k = 0; lane_path && k < L.cols.size(); ++k) lane_path = nprim_of(cd[L.cols[k]]) == 1;
     if (!lane_path) trans = false;
+    // s-s side: the thin-ket 4x4x4 kernel with the s-s pairs as its ket (one
+    // column each, sixteen quartets per instruction)
+    bool thin = false;
+    if (!generic && R.mode == LAUNCH_PRODUCT && C.eri_md4 && std::min(L.E, L.F) == 0 &&
+        std::max(L.E, L.F) >= C.eri_thin_emin) {
+      const bool tt = L.F != 0;   // E = 0 < F: the caller's rows are the s-s side
+      thin = !tt || C.eri_trans;
+      for (int c : (tt ? L.rows : L.cols)) thin = thin && nrow_of((tt ? rd : cd)[c]) == 1.0;
+      if (thin) {
+        lane_path = false;
+        trans = tt;
+      }
+    }
     // MFMA kernel orientation: its bra side (stage-2 rows, 16-row tiles) is
     // the side with more rows per pair; the thin side is stacked in columns
-    if (!lane_path && !generic && R.mode == LAUNCH_PRODUCT && C.eri_trans) {
+    if (!thin && !lane_path && !generic && R.mode == LAUNCH_PRODUCT && C.eri_trans) {
       double sr = 0, sc = 0;
       for (int r : L.rows) sr += nrow_of(rd[r]);
       for (int c : L.cols) sc += nrow_of(cd[c]);
@@ -451,7 +461,7 @@ void eri_run(Context &C, const EriRequest &R) {
     // instruction count -- per tile and bra primitive pair, NHG (KSF ket
     // primitives + ceil(nr / 4)) instructions, plus the R tables (about one
     // instruction's time per 40 recursion entries)
-    bool use4 = !lane_path && !generic && R.mode == LAUNCH_PRODUCT && C.eri_md4;
+    bool use4 = !thin && !lane_path && !generic && R.mode == LAUNCH_PRODUCT && C.eri_md4;
     std::vector<ColTile> t4[2];
     if (use4) {
       double best = 0;
@@ -541,7 +551,11 @@ void eri_run(Context &C, const EriRequest &R) {
     a.boys = C.dboys.p;
     long long grid;
     long long ct0 = 0, nct = 0;
-    if (use4) {
+    if (thin) {
+      a.ntiles = (nrows + M4_NB - 1) / M4_NB * ((ncols + M4S_NK - 1) / M4S_NK);
+      grid = std::min<long long>(a.ntiles, 8192);
+      if (grid >= 8) grid = (grid + 7) / 8 * 8;
+    } else if (use4) {
       std::vector<ColTile> &tl = t4[trans ? 1 : 0];
       ct0 = (long long)ctiles.size();
       nct = (long long)tl.size();
@@ -574,14 +588,65 @@ void eri_run(Context &C, const EriRequest &R) {
     bool plain = true;
     for (size_t k = 0; plain && k < L.rows.size(); ++k) plain = nprim_of(rd[L.rows[k]]) == 1;
     for (size_t k = 0; plain && k < L.cols.size(); ++k) plain = nprim_of(cd[L.cols[k]]) == 1;
-    prep.push_back(Prepared{a, grid, use4 ? 3 : (generic ? 2 : (lane_path ? 1 : 0)), KE, KF, L.E, L.F, nq, fl,
-                            ct0, nct, plain});
+    prep.push_back(Prepared{a, grid, thin ? 4 : (use4 ? 3 : (generic ? 2 : (lane_path ? 1 : 0))), KE, KF, L.E, L.F,
+                            nq, fl, ct0, nct, plain});
     nq_total += nq;
     flops_total += fl;
   }
+  // the request's host tables (pair-table jobs, descriptors, launch index
+  // lists, column tiles) go to the device in one copy from pinned memory;
+  // the kernels read them from that one device block
+  const EtJob *d_jobs;
+  const PairDesc *d_rd, *d_cd;
+  const int *d_idx;
+  const unsigned char *d_side;
+  const ColTile *d_ct;
+  {
+    auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    const size_t o_jobs = 0;
+    const size_t o_rd = o_jobs + al(jobs.size() * sizeof(EtJob));
+    const size_t o_cd = o_rd + al(rd.size() * sizeof(PairDesc));
+    const size_t o_idx = o_cd + al(cd.size() * sizeof(PairDesc));
+    const size_t o_side = o_idx + al(idx.size() * sizeof(int));
+    const size_t o_ct = o_side + al(side.size());
+    const size_t nbytes = o_ct + al(ctiles.size() * sizeof(ColTile));
+    unsigned char *h = C.stage_host(nbytes);
+    auto put = [&](size_t o, const void *src, size_t n) {
+      if (n) memcpy(h + o, src, n);
+    };
+    put(o_jobs, jobs.data(), jobs.size() * sizeof(EtJob));
+    put(o_rd, rd.data(), rd.size() * sizeof(PairDesc));
+    put(o_cd, cd.data(), cd.size() * sizeof(PairDesc));
+    put(o_idx, idx.data(), idx.size() * sizeof(int));
+    put(o_side, side.data(), side.size());
+    put(o_ct, ctiles.data(), ctiles.size() * sizeof(ColTile));
+    C.md_blob.alloc(nbytes);
+    GI_HIP(hipMemcpyAsync(C.md_blob.p, h, nbytes, hipMemcpyHostToDevice, C.stream));
+    GI_HIP(hipEventRecord(C.md_stage_ev, C.stream));   // the next stage_host waits for it
+    unsigned char *b = C.md_blob.p;
+    d_jobs = (const EtJob *)(b + o_jobs);
+    d_rd = (const PairDesc *)(b + o_rd);
+    d_cd = (const PairDesc *)(b + o_cd);
+    d_idx = (const int *)(b + o_idx);
+    d_side = (const unsigned char *)(b + o_side);
+    d_ct = (const ColTile *)(b + o_ct);
+  }
+  // the timed window (when asked for) spans the request's kernels only:
+  // pair tables, record packing and the class launches
+  if (C.time_eri) GI_HIP(hipEventRecord(e0, C.stream));
+  if (!jobs.empty()) {
+    hipLaunchKernelGGL(k_build_et, dim3((unsigned)jobs.size()), dim3(64), 0, C.stream, C.dshells.p, C.dexps.p,
+                       C.dcntr.p, C.dnorms.p, C.spher, d_jobs, C.md_prim.p, C.md_et.p);
+    GI_HIP(hipGetLastError());
+    ++nlaunch;
+  }
+  if (!idx.empty()) {
+    hipLaunchKernelGGL(k_pack_recs, dim3((unsigned)((idx.size() + 255) / 256)), dim3(256), 0, C.stream, d_rd, d_cd,
+                       d_idx, d_side, (long long)idx.size(), C.md_hdr.p, C.md_prim.p, C.md_rec.p);
+    GI_HIP(hipGetLastError());   // record packing: not counted among the ERI kernel launches
+  }
   // per-class statistics time every launch on its own, so they stay on the
   // context stream; otherwise the launches are spread over the side streams
-  if (!ctiles.empty()) C.md_ctile.upload(ctiles, C.stream);
   const int nstream = stats ? 1 : (int)std::min<long long>(C.eri_streams, (long long)prep.size());
   if (nstream > 1) {
     if (!C.eri_fork) {
@@ -609,8 +674,10 @@ void eri_run(Context &C, const EriRequest &R) {
       GI_HIP(hipEventCreate(&s1));
       GI_HIP(hipEventRecord(s0, st));
     }
-    if (P.kind == 3)
-      md4_table[P.KE][P.KF](P.a, C.md_ctile.p + P.ct0, P.nct, P.plain, (int)P.grid, st);
+    if (P.kind == 4)
+      md4s_table[P.KE](P.a, P.plain, (int)P.grid, st);
+    else if (P.kind == 3)
+      md4_table[P.KE][P.KF](P.a, d_ct + P.ct0, P.nct, P.plain, (int)P.grid, st);
     else if (P.kind == 2)
       launch_generic(P.KE, P.KF, P.a, st);
     else

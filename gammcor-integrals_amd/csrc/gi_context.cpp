@@ -38,15 +38,10 @@ Context::~Context() {
   dnorms.release();
   dboys.release();
   md_hdr.release();
-  md_jobs.release();
   md_prim.release();
   md_et.release();
-  md_rdesc.release();
-  md_cdesc.release();
-  md_idx.release();
-  md_side.release();
+  md_blob.release();
   md_rec.release();
-  md_ctile.release();
   chol.inv_l.release();
   chol.rkpq.release();
   chol.mo_adj_off.release();
@@ -66,6 +61,22 @@ Context::~Context() {
   }
   if (eri_fork) (void)hipEventDestroy(eri_fork);
   if (stream) (void)hipStreamDestroy(stream);
+  if (md_stage) (void)hipHostFree(md_stage);
+  if (md_stage_ev) (void)hipEventDestroy(md_stage_ev);
+}
+
+unsigned char *Context::stage_host(size_t n) {
+  if (!md_stage_ev) GI_HIP(hipEventCreateWithFlags(&md_stage_ev, hipEventDisableTiming));
+  else GI_HIP(hipEventSynchronize(md_stage_ev));
+  if (n <= md_stage_cap && md_stage) return md_stage;
+  if (md_stage) {
+    GI_HIP(hipHostFree(md_stage));
+    md_stage = nullptr;
+  }
+  const size_t cap = std::max(n, md_stage_cap + md_stage_cap / 2);
+  GI_HIP(hipHostMalloc((void **)&md_stage, cap, hipHostMallocDefault));
+  md_stage_cap = cap;
+  return md_stage;
 }
 
 }  // namespace gi

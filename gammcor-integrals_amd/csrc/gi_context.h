@@ -108,15 +108,18 @@ struct Context {
   std::vector<PairHdr> md_hdr_host;
   long long md_et_used = 0, md_prim_used = 0;
   DBuf<PairHdr> md_hdr;
-  DBuf<EtJob> md_jobs;
   DBuf<PrimRec> md_prim;
   DBuf<double> md_et;
-  // per request: descriptors, launch index lists, packed records
-  DBuf<PairDesc> md_rdesc, md_cdesc;
-  DBuf<int> md_idx;
-  DBuf<unsigned char> md_side;
+  // per request: one device block with the pair-table jobs, descriptors,
+  // launch index lists and 4x4x4 column tiles (copied from a pinned host
+  // staging buffer), and the packed pair records
+  DBuf<unsigned char> md_blob;
   DBuf<QRec> md_rec;
-  DBuf<ColTile> md_ctile;   // column tiles of the request's 4x4x4 launches
+  unsigned char *md_stage = nullptr;   // pinned (hipHostMalloc), grow-only
+  size_t md_stage_cap = 0;
+  hipEvent_t md_stage_ev = nullptr;    // recorded after each copy out of md_stage
+  // pinned staging of at least n bytes, once the last copy out of it is done
+  unsigned char *stage_host(size_t n);
   void reset_pair_cache() {
     md_pt.clear();
     md_hdr_host.clear();
@@ -142,6 +145,10 @@ struct Context {
   long long eri_min_tiles = 8192, eri_max_kt = 32;
   bool eri_lane = true, eri_trans = true;
   bool eri_md4 = true;   // products on the 4x4x4 kernel (md4_kernel.h) instead of md_kernel
+  // products with an s-s side (E = 0 or F = 0) whose other side has order >= this
+  // run on the thin-ket form of the 4x4x4 kernel (7: never); below 2 the lane
+  // kernels are faster (measured on the (H2O)16 decomposition)
+  long long eri_thin_emin = 2;
   long long eri_lane_emax = LANE_EMAX;   // lane kernels only for classes with E, F <= this
   long long vec_blocks = 20;             // block rows of the triangular Inv_L Vabrs^T product
   // class launches of one ERI request are spread over this many streams (the

@@ -358,4 +358,209 @@ __global__ void __launch_bounds__(64) MD4_ATTR md4_kernel(MdArgs A, const ColTil
   }
 }
 
+// ---------------------------------------------------------------------------
+// Thin-ket variant: classes whose ket pairs are s-s pairs (F = 0, one Hermite
+// function and one column per ket pair).  Stage 1 reduces to
+// X(h, c) = R_h(quartet of c) ET_ket(c), a scale instead of a product, so the
+// four blocks of an instruction need not share a quartet: a tile is M4_NB bra
+// pairs x M4S_NK consecutive ket pairs, every column its own quartet, and
+// stage 2 (the bra table, shared by all blocks, times X) fills all sixteen
+// columns of the instruction.  The R tables are packed (index h, no h + k
+// addressing), sixteen per bra pair, four lanes per quartet; even recursion
+// levels in one table, odd levels in the other, level 0 last.  The Boys
+// values of the tile's quartets stay in the registers of the lane that
+// evaluated them and reach the recursion lanes by cross-lane reads.
+constexpr int M4S_NK = 16;
+
+template <int E, bool PLAIN>
+__global__ void __launch_bounds__(64) MD4_ATTR md4s_kernel(MdArgs A) {
+  constexpr int L = E, NHE = nherm(E), NHG = (NHE + 3) / 4;
+  constexpr int NL1 = L >= 1 ? nherm(L - 1) : 1;
+  constexpr int SB = NHE | 1, SA = NL1 | 1;   // odd row strides (LDS banks)
+  constexpr int NRMAX = nab_max(E);
+  __shared__ double s_Rb[M4S_NK * SB];        // even levels, level 0 last
+  __shared__ double s_Ra[M4S_NK * SA];        // odd levels
+  __shared__ double s_pq[M4S_NK][4];          // P - Q of the bra pair's quartets
+  __shared__ int s_rrec[NHE];
+  __shared__ unsigned s_frow[NRMAX + 4];
+
+  const int lane = threadIdx.x;
+  const int lq = lane >> 4, li = lane & 3, col = lane & 15;
+  for (int e = lane; e < NHE; e += 64) s_rrec[e] = c_rrec[e];
+  __syncthreads();
+  const bool hlast_ok = 4 * (NHG - 1) + lq < NHE;
+  constexpr bool hg_last_mask = (NHE % 4) != 0;
+
+  const int G = gridDim.x;
+  const int w = blockIdx.x;
+  const long long v0 = (G % 8 == 0) ? (long long)(w % 8) * (G / 8) + w / 8 : w;
+  const long long rs = A.trans ? A.ldo : 1;
+  const long long nrb = (A.nrows + M4_NB - 1) / M4_NB;
+  const long long nct = (A.ncols + M4S_NK - 1) / M4S_NK;
+  const long long ntiles = nrb * nct;
+
+  // the Boys values (scaled F_0 .. F_L) and P - Q of this lane's quartet
+  double Fq[L + 1], PQq[3];
+  auto clear_boys = [&]() {
+#pragma unroll
+    for (int n = 0; n <= L; ++n) Fq[n] = 0.0;
+    PQq[0] = PQq[1] = PQq[2] = 0.0;
+  };
+  // R tables of the sixteen quartets whose Boys values sit in lanes
+  // src0 .. src0 + 15: lanes 4 j .. 4 j + 3 recurse quartet j
+  auto build_r = [&](int src0) {
+    const int j = lane >> 2, sub = lane & 3;
+    const int src = src0 + j;
+    const double px_ = __shfl(PQq[0], src), py_ = __shfl(PQq[1], src), pz_ = __shfl(PQq[2], src);
+    const double ftop = __shfl(Fq[L], src);
+    __syncthreads();   // earlier readers of the tables are done
+    if (sub == 0) {
+      s_pq[j][0] = px_;
+      s_pq[j][1] = py_;
+      s_pq[j][2] = pz_;
+      ((L & 1) ? s_Ra + j * SA : s_Rb + j * SB)[0] = ftop;
+    }
+#pragma unroll
+    for (int n = L - 1; n >= 0; --n) {
+      const double fn = __shfl(Fq[n], src);
+      __syncthreads();
+      const int cnt = nherm(L - n);
+      const double *prv = ((n + 1) & 1) ? s_Ra + j * SA : s_Rb + j * SB;
+      double *cur = (n & 1) ? s_Ra + j * SA : s_Rb + j * SB;
+      for (int idx = sub; idx < cnt; idx += 4) {
+        double r;
+        if (idx == 0) {
+          r = fn;
+        } else {
+          const int rec = s_rrec[idx];
+          const int i1 = rec & 0x7ff, i2 = (rec >> 11) & 0x7ff, dir = (rec >> 22) & 3, coef = rec >> 24;
+          r = (double)coef * prv[i2] + s_pq[j][dir] * prv[i1];
+        }
+        cur[idx] = r;
+      }
+    }
+    __syncthreads();
+  };
+  // stage 1: X(h, c) (+)= R_h ET_ket(c) for this lane's h = 4 hg + lq, c = col
+  double X[NHG];
+  auto stage1 = [&](double etv, bool add) {
+    const double *Rc = s_Rb + col * SB;
+#pragma unroll
+    for (int hg = 0; hg < NHG; ++hg) {
+      const int h = 4 * hg + lq;
+      const double v = Rc[h < NHE ? h : NHE - 1] * etv;
+      const double xv = (hg == NHG - 1 && hg_last_mask && !hlast_ok) ? 0.0 : v;
+      X[hg] = add ? X[hg] + xv : xv;
+    }
+  };
+  auto fill_rows = [&](const QRec &RB) {
+    for (int r = lane; r < RB.nrow; r += 64) s_frow[r] = 8u * (unsigned)pair_full(RB, A.sel, r);
+    if (lane < 4) s_frow[RB.nrow + lane] = 0u;
+    __syncthreads();
+  };
+  // stage 2 as in md4_kernel: V(r, c) = sum_h ET_bra(h, r) X(h, c), column c
+  // of this lane the single column of its ket pair
+  auto stage2 = [&](const QRec &RB, const double *etb, double *outc, bool cval, bool accumulate) {
+    const int nr = RB.nrow, ldb = RB.n0 * RB.n1;
+    const __amdgpu_buffer_rsrc_t ub = make_rsrc(uniform_ptr(etb), 8u * (unsigned)(NHE * ldb));
+    const unsigned hstep = 32u * (unsigned)ldb;
+    const unsigned lrow = 8u * (unsigned)(lq * ldb);
+    const unsigned lrow_last = 8u * (unsigned)((imin(4 * (NHG - 1) + lq, NHE - 1) - 4 * (NHG - 1)) * ldb);
+    double *const out = outc + RB.off;
+    const int nrg = (nr + 3) / 4;
+    double ea[NHG];
+    auto load_rows = [&](int rg) {
+      const unsigned fro = s_frow[4 * rg + li];
+#pragma unroll
+      for (int hg = 0; hg < NHG; ++hg) {
+        const unsigned o = (hg == NHG - 1 && (NHE % 4) != 0) ? lrow_last + fro : lrow + fro;
+        ea[hg] = buf_ld(ub, o, (unsigned)hg * hstep);
+      }
+    };
+    load_rows(0);
+    for (int rg = 0; rg < nrg; ++rg) {
+      double e[NHG];
+#pragma unroll
+      for (int hg = 0; hg < NHG; ++hg) e[hg] = ea[hg];
+      if (hg_last_mask) e[NHG - 1] = hlast_ok ? e[NHG - 1] : 0.0;
+      if (rg + 1 < nrg) load_rows(rg + 1);
+      double v0 = 0.0, v1 = 0.0;
+#pragma unroll
+      for (int hg = 0; hg < NHG; ++hg) {
+        if (hg & 1)
+          v1 = __builtin_amdgcn_mfma_f64_4x4x4f64(e[hg], X[hg], v1, 0, 0, 0);
+        else
+          v0 = __builtin_amdgcn_mfma_f64_4x4x4f64(e[hg], X[hg], v0, 0, 0, 0);
+      }
+      const int ro = 4 * rg + lq;
+      if (cval && ro < nr) {
+        double *d = out + rs * ro;
+        *d = accumulate ? *d + (v0 + v1) : v0 + v1;
+      }
+    }
+  };
+
+  for (long long t = v0; t < ntiles; t += G) {
+    const long long rb = t / nct, ct = t - rb * nct;
+    const long long ir0 = rb * M4_NB, jc0 = ct * M4S_NK;
+    const int nb = (int)((A.nrows - ir0) < M4_NB ? (A.nrows - ir0) : M4_NB);
+    const int nk = (int)((A.ncols - jc0) < M4S_NK ? (A.ncols - jc0) : M4S_NK);
+    // this lane's column: ket pair jc0 + col, its single orbital pair
+    const bool cval = col < nk;
+    const QRec CD = A.crec[jc0 + (cval ? col : 0)];
+    const int fc = cval ? pair_full(CD, A.sel, 0) : 0;
+    double *const outc = A.out + CD.off;
+
+    if constexpr (PLAIN) {
+      // one Boys pass for every quartet of the tile: lane 16 b + j takes
+      // bra pair b, ket pair j
+      {
+        const int b = lane >> 4, j = lane & 15;
+        clear_boys();
+        if (b < nb && j < nk)
+          md4_boys<L>(first_prim(A.rrec[ir0 + b]), first_prim(A.crec[jc0 + j]), A.kappa, A.boys, Fq, PQq);
+      }
+      const double etv = cval ? A.et[CD.et + fc] : 0.0;
+      for (int b = 0; b < nb; ++b) {
+        const QRec RB = A.rrec[ir0 + b];
+        fill_rows(RB);
+        build_r(16 * b);
+        stage1(etv, false);
+        stage2(RB, A.et + RB.et, outc, cval, false);
+      }
+    } else {
+      // contracted pairs: per bra pair and bra primitive pair, X summed over
+      // the ket primitive pairs (lanes 0 .. 15 evaluate the Boys values of the
+      // sixteen quartets of one primitive step)
+      int maxnp = 1;
+      for (int j = 0; j < nk; ++j) maxnp = imax(maxnp, A.crec[jc0 + j].nprim);
+      for (int b = 0; b < nb; ++b) {
+        const QRec RB = A.rrec[ir0 + b];
+        fill_rows(RB);
+        for (int bp = 0; bp < RB.nprim; ++bp) {
+          const PrimRec PB = bp == 0 ? first_prim(RB) : A.prim[RB.prim0 + bp];
+          for (int kp = 0; kp < maxnp; ++kp) {
+            clear_boys();
+            if (lane < nk) {
+              const QRec CQ = A.crec[jc0 + lane];
+              if (kp < CQ.nprim) {
+                const PrimRec PK = kp == 0 ? first_prim(CQ) : A.prim[CQ.prim0 + kp];
+                md4_boys<L>(PB, PK, A.kappa, A.boys, Fq, PQq);
+              }
+            }
+            build_r(0);
+            double etv = 0.0;
+            if (cval && kp < CD.nprim) {
+              const PrimRec PK = kp == 0 ? first_prim(CD) : A.prim[CD.prim0 + kp];
+              etv = A.et[PK.et + fc];
+            }
+            stage1(etv, kp > 0);
+          }
+          stage2(RB, A.et + PB.et, outc, cval, bp > 0);
+        }
+      }
+    }
+  }
+}
+
 }  // namespace gi

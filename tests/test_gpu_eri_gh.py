@@ -91,3 +91,53 @@ def test_cholesky_with_g_and_h_shells(gpu_ctx, oracle_mod):
     assert np.abs(Gg - Gr).max() <= 1e-11 * np.abs(Gr).max()
     rmax, _ = gpu_ctx.chol_residual()
     assert rmax <= 1e-6 * (1 + 1e-6)
+
+
+def _pair_block_vs_oracle(b, o, V, rp, cp, tol):
+    nang = b.shell_nang
+    r0, worst = 0, 0.0
+    for (A, B) in rp:
+        nr = nang(A) * nang(B)
+        c0 = 0
+        for (Cs, D) in cp:
+            nc = nang(Cs) * nang(D)
+            ref = o.eri(A, B, Cs, D).transpose(1, 0, 3, 2).reshape(nr, nc)
+            err = float(np.abs(V[r0:r0 + nr, c0:c0 + nc] - ref).max())
+            worst = max(worst, err)
+            assert err <= tol, ((A, B), (Cs, D), err)
+            c0 += nc
+        r0 += nr
+    return worst
+
+
+@pytest.mark.parametrize("contracted", [False, True], ids=["plain", "contracted"])
+@pytest.mark.parametrize("thin_emin", [0, 2, 7], ids=["thin-all", "thin-default", "thin-off"])
+def test_thin_ket_pair_blocks(gpu_ctx, oracle_mod, contracted, thin_emin):
+    """Products with an s-s side: the thin-ket 4x4x4 kernel (sixteen s-s kets
+    per tile, one quartet per column) against the oracle, with the s-s pairs
+    as the caller's columns and as its rows (transposed launch), bra orders
+    0 .. 6, more s-s pairs than one tile holds."""
+    b = systems.water_system(2, 6.0, "gh", seed=13, contracted=contracted)
+    o = oracle_mod.Oracle(b)
+    gpu_ctx.set_basis(b)
+    by_l = _by_l(b)
+    rng = np.random.default_rng(17)
+    ss = sorted({tuple(sorted(map(int, rng.choice(by_l[0], 2)), reverse=True)) for _ in range(60)})[:37]
+    thick = []
+    for l0, l1 in [(0, 0), (1, 0), (1, 1), (2, 0), (2, 1), (3, 0), (2, 2), (4, 0), (3, 1), (3, 2), (5, 0), (4, 1),
+                   (3, 3), (4, 2), (5, 1)]:
+        for _ in range(2):
+            s0, s1 = int(rng.choice(by_l[l0])), int(rng.choice(by_l[l1]))
+            thick.append((s0, s1) if s0 >= s1 else (s1, s0))
+    tol = TOL_CONTRACTED if contracted else TOL
+    try:
+        gpu_ctx.set_option("eri_min_tiles", 1)
+        gpu_ctx.set_option("eri_thin_emin", thin_emin)
+        V1 = gpu_ctx.eri_pair_block(thick, ss)
+        V2 = gpu_ctx.eri_pair_block(ss, thick)
+    finally:
+        gpu_ctx.set_option("eri_min_tiles", 8192)
+        gpu_ctx.set_option("eri_thin_emin", 2)
+    w1 = _pair_block_vs_oracle(b, o, V1, thick, ss, tol)
+    w2 = _pair_block_vs_oracle(b, o, V2, ss, thick, tol)
+    print("thin_emin %d: max |gpu - oracle| = %.2e (s-s columns), %.2e (s-s rows)" % (thin_emin, w1, w2))
