@@ -551,8 +551,17 @@ void eri_run(Context &C, const EriRequest &R) {
     a.boys = C.dboys.p;
     long long grid;
     long long ct0 = 0, nct = 0;
+    // 4x4x4 kernels: row blocks of M4_NB bra pairs, or fewer when that leaves
+    // fewer than eri_md4_tiles tiles (small launches: more, shorter wavefronts)
+    auto row_block = [&](long long ncolt) {
+      int nb = M4_NB;
+      while (nb > 1 && (nrows + nb - 1) / nb * ncolt < C.eri_md4_tiles) nb /= 2;
+      return nb;
+    };
     if (thin) {
-      a.ntiles = (nrows + M4_NB - 1) / M4_NB * ((ncols + M4S_NK - 1) / M4S_NK);
+      const long long ncolt = (ncols + M4S_NK - 1) / M4S_NK;
+      a.kt = row_block(ncolt);
+      a.ntiles = (nrows + a.kt - 1) / a.kt * ncolt;
       grid = std::min<long long>(a.ntiles, 8192);
       if (grid >= 8) grid = (grid + 7) / 8 * 8;
     } else if (use4) {
@@ -560,7 +569,8 @@ void eri_run(Context &C, const EriRequest &R) {
       ct0 = (long long)ctiles.size();
       nct = (long long)tl.size();
       ctiles.insert(ctiles.end(), tl.begin(), tl.end());
-      a.ntiles = (nrows + M4_NB - 1) / M4_NB * nct;   // row blocks of M4_NB bra pairs x column tiles
+      a.kt = row_block(nct);
+      a.ntiles = (nrows + a.kt - 1) / a.kt * nct;   // row blocks x column tiles
       grid = std::min<long long>(a.ntiles, 8192);
       if (grid >= 8) grid = (grid + 7) / 8 * 8;
     } else if (lane_path) {

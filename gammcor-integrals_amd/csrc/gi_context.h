@@ -149,6 +149,9 @@ struct Context {
   // run on the thin-ket form of the 4x4x4 kernel (7: never); below 2 the lane
   // kernels are faster (measured on the (H2O)16 decomposition)
   long long eri_thin_emin = 2;
+  // 4x4x4 launches take four bra pairs per tile, or two or one when four would
+  // leave fewer tiles than this
+  long long eri_md4_tiles = 2048;
   long long eri_lane_emax = LANE_EMAX;   // lane kernels only for classes with E, F <= this
   long long vec_blocks = 20;             // block rows of the triangular Inv_L Vabrs^T product
   // class launches of one ERI request are spread over this many streams (the

@@ -168,7 +168,10 @@ __global__ void __launch_bounds__(64) MD4_ATTR md4_kernel(MdArgs A, const ColTil
   const int w = blockIdx.x;
   const long long v0 = (G % 8 == 0) ? (long long)(w % 8) * (G / 8) + w / 8 : w;
   const long long rs = A.trans ? A.ldo : 1, cs = A.trans ? 1 : A.ldo;
-  const long long nrb = (A.nrows + M4_NB - 1) / M4_NB;
+  // row blocks of A.kt (1 .. M4_NB) bra pairs: fewer per tile where a launch
+  // has too few tiles to fill the device
+  const int nbr = A.kt;
+  const long long nrb = (A.nrows + nbr - 1) / nbr;
   const long long ntiles = nrb * nct;
 
   // R tables of the nq quartets of bra pair b (Boys values in s_F[b], s_PQ[b]):
@@ -208,8 +211,8 @@ __global__ void __launch_bounds__(64) MD4_ATTR md4_kernel(MdArgs A, const ColTil
 
   for (long long t = v0; t < ntiles; t += G) {
     const long long rb = t / nct, ct = t - rb * nct;
-    const long long ir0 = rb * M4_NB;
-    const int nb = (int)((A.nrows - ir0) < M4_NB ? (A.nrows - ir0) : M4_NB);
+    const long long ir0 = rb * nbr;
+    const int nb = (int)((A.nrows - ir0) < nbr ? (A.nrows - ir0) : nbr);
     const ColTile T = ctiles[ct];
     const int nq = T.nq;
     // this lane's slot: ket pair, column (shared by the tile's bra pairs)
@@ -261,28 +264,34 @@ __global__ void __launch_bounds__(64) MD4_ATTR md4_kernel(MdArgs A, const ColTil
     };
     // stage 2 for one bra primitive pair: V(r, c) = sum_h ET_bra(h, r) X(h, c),
     // four bra rows per step (the A operand's row 4 rg + li); the table of h step
-    // hg starts at a uniform address, each lane reads at a 32-bit offset; two
-    // accumulator chains (even and odd h steps)
-    auto stage2 = [&](const QRec &RB, const double *etb, bool accumulate) {
-      const int nr = RB.nrow, ldb = RB.n0 * RB.n1;
-      const __amdgpu_buffer_rsrc_t ub = make_rsrc(uniform_ptr(etb), 8u * (unsigned)(NHE * ldb));
-      const unsigned hstep = 32u * (unsigned)ldb;   // 4 h rows, bytes
-      const unsigned lrow = 8u * (unsigned)(lq * ldb);
+    // hg starts at a uniform address, each lane reads at a 32-bit offset.
+    // s2_open + load_rows(0) issue the first row group's loads as soon as the
+    // bra rows are known, so that they complete under the R recursion and
+    // stage 1; each row group then loads the next one while it multiplies.
+    __amdgpu_buffer_rsrc_t s2_ub = make_rsrc(uniform_ptr(A.et), 0u);
+    unsigned s2_hstep = 0, s2_lrow = 0, s2_lrow_last = 0;
+    double ea[NHG];
+    auto s2_open = [&](const QRec &RB, const double *etb) {
+      const int ldb = RB.n0 * RB.n1;
+      s2_ub = make_rsrc(uniform_ptr(etb), 8u * (unsigned)(NHE * ldb));
+      s2_hstep = 32u * (unsigned)ldb;   // 4 h rows, bytes
+      s2_lrow = 8u * (unsigned)(lq * ldb);
       // the last h step reads row min(h, NHE - 1) (inside the block), masked
-      const unsigned lrow_last = 8u * (unsigned)((imin(4 * (NHG - 1) + lq, NHE - 1) - 4 * (NHG - 1)) * ldb);
+      s2_lrow_last = 8u * (unsigned)((imin(4 * (NHG - 1) + lq, NHE - 1) - 4 * (NHG - 1)) * ldb);
+    };
+    auto load_rows = [&](int rg) {
+      const unsigned fro = s_frow[4 * rg + li];
+#pragma unroll
+      for (int hg = 0; hg < NHG; ++hg) {
+        const unsigned o = (hg == NHG - 1 && (NHE % 4) != 0) ? s2_lrow_last + fro : s2_lrow + fro;
+        ea[hg] = buf_ld(s2_ub, o, (unsigned)hg * s2_hstep);
+      }
+    };
+    // two accumulator chains (even and odd h steps)
+    auto stage2 = [&](const QRec &RB, bool accumulate) {
+      const int nr = RB.nrow;
       double *const out = A.out + RB.off + CD.off;
       const int nrg = (nr + 3) / 4;
-      // the operands of row group rg + 1 are read while rg multiplies
-      double ea[NHG];
-      auto load_rows = [&](int rg) {
-        const unsigned fro = s_frow[4 * rg + li];
-#pragma unroll
-        for (int hg = 0; hg < NHG; ++hg) {
-          const unsigned o = (hg == NHG - 1 && (NHE % 4) != 0) ? lrow_last + fro : lrow + fro;
-          ea[hg] = buf_ld(ub, o, (unsigned)hg * hstep);
-        }
-      };
-      load_rows(0);
       for (int rg = 0; rg < nrg; ++rg) {
         double e[NHG];
 #pragma unroll
@@ -319,11 +328,13 @@ __global__ void __launch_bounds__(64) MD4_ATTR md4_kernel(MdArgs A, const ColTil
       for (int b = 0; b < nb; ++b) {
         const QRec RB = A.rrec[ir0 + b];
         fill_rows(RB);
+        s2_open(RB, A.et + RB.et);
+        load_rows(0);
         build_r(b, nq, 0xfu);
 #pragma unroll
         for (int hg = 0; hg < NHG; ++hg) X[hg] = 0.0;
         stage1();
-        stage2(RB, A.et + RB.et, false);
+        stage2(RB, false);
       }
     } else {
       // contracted pairs: per bra pair and bra primitive pair, X summed over
@@ -335,6 +346,8 @@ __global__ void __launch_bounds__(64) MD4_ATTR md4_kernel(MdArgs A, const ColTil
         fill_rows(RB);
         for (int bp = 0; bp < RB.nprim; ++bp) {
           const PrimRec PB = bp == 0 ? first_prim(RB) : A.prim[RB.prim0 + bp];
+          s2_open(RB, A.et + PB.et);
+          load_rows(0);
 #pragma unroll
           for (int hg = 0; hg < NHG; ++hg) X[hg] = 0.0;
           for (int kp = 0; kp < maxnp; ++kp) {
@@ -351,7 +364,7 @@ __global__ void __launch_bounds__(64) MD4_ATTR md4_kernel(MdArgs A, const ColTil
             load_eb(kp);
             stage1();
           }
-          stage2(RB, A.et + PB.et, bp > 0);
+          stage2(RB, bp > 0);
         }
       }
     }
@@ -362,8 +375,8 @@ __global__ void __launch_bounds__(64) MD4_ATTR md4_kernel(MdArgs A, const ColTil
 // Thin-ket variant: classes whose ket pairs are s-s pairs (F = 0, one Hermite
 // function and one column per ket pair).  Stage 1 reduces to
 // X(h, c) = R_h(quartet of c) ET_ket(c), a scale instead of a product, so the
-// four blocks of an instruction need not share a quartet: a tile is M4_NB bra
-// pairs x M4S_NK consecutive ket pairs, every column its own quartet, and
+// four blocks of an instruction need not share a quartet: a tile is up to
+// M4_NB bra pairs x M4S_NK consecutive ket pairs, every column its own quartet, and
 // stage 2 (the bra table, shared by all blocks, times X) fills all sixteen
 // columns of the instruction.  The R tables are packed (index h, no h + k
 // addressing), sixteen per bra pair, four lanes per quartet; even recursion
@@ -395,7 +408,8 @@ __global__ void __launch_bounds__(64) MD4_ATTR md4s_kernel(MdArgs A) {
   const int w = blockIdx.x;
   const long long v0 = (G % 8 == 0) ? (long long)(w % 8) * (G / 8) + w / 8 : w;
   const long long rs = A.trans ? A.ldo : 1;
-  const long long nrb = (A.nrows + M4_NB - 1) / M4_NB;
+  const int nbr = A.kt;   // bra pairs per row block, 1 .. M4_NB
+  const long long nrb = (A.nrows + nbr - 1) / nbr;
   const long long nct = (A.ncols + M4S_NK - 1) / M4S_NK;
   const long long ntiles = nrb * nct;
 
@@ -458,26 +472,31 @@ __global__ void __launch_bounds__(64) MD4_ATTR md4s_kernel(MdArgs A) {
     if (lane < 4) s_frow[RB.nrow + lane] = 0u;
     __syncthreads();
   };
-  // stage 2 as in md4_kernel: V(r, c) = sum_h ET_bra(h, r) X(h, c), column c
-  // of this lane the single column of its ket pair
-  auto stage2 = [&](const QRec &RB, const double *etb, double *outc, bool cval, bool accumulate) {
-    const int nr = RB.nrow, ldb = RB.n0 * RB.n1;
-    const __amdgpu_buffer_rsrc_t ub = make_rsrc(uniform_ptr(etb), 8u * (unsigned)(NHE * ldb));
-    const unsigned hstep = 32u * (unsigned)ldb;
-    const unsigned lrow = 8u * (unsigned)(lq * ldb);
-    const unsigned lrow_last = 8u * (unsigned)((imin(4 * (NHG - 1) + lq, NHE - 1) - 4 * (NHG - 1)) * ldb);
+  // stage 2 as in md4_kernel (first row group loaded ahead): V(r, c) =
+  // sum_h ET_bra(h, r) X(h, c), column c of this lane the single column of
+  // its ket pair
+  __amdgpu_buffer_rsrc_t s2_ub = make_rsrc(uniform_ptr(A.et), 0u);
+  unsigned s2_hstep = 0, s2_lrow = 0, s2_lrow_last = 0;
+  double ea[NHG];
+  auto s2_open = [&](const QRec &RB, const double *etb) {
+    const int ldb = RB.n0 * RB.n1;
+    s2_ub = make_rsrc(uniform_ptr(etb), 8u * (unsigned)(NHE * ldb));
+    s2_hstep = 32u * (unsigned)ldb;
+    s2_lrow = 8u * (unsigned)(lq * ldb);
+    s2_lrow_last = 8u * (unsigned)((imin(4 * (NHG - 1) + lq, NHE - 1) - 4 * (NHG - 1)) * ldb);
+  };
+  auto load_rows = [&](int rg) {
+    const unsigned fro = s_frow[4 * rg + li];
+#pragma unroll
+    for (int hg = 0; hg < NHG; ++hg) {
+      const unsigned o = (hg == NHG - 1 && (NHE % 4) != 0) ? s2_lrow_last + fro : s2_lrow + fro;
+      ea[hg] = buf_ld(s2_ub, o, (unsigned)hg * s2_hstep);
+    }
+  };
+  auto stage2 = [&](const QRec &RB, double *outc, bool cval, bool accumulate) {
+    const int nr = RB.nrow;
     double *const out = outc + RB.off;
     const int nrg = (nr + 3) / 4;
-    double ea[NHG];
-    auto load_rows = [&](int rg) {
-      const unsigned fro = s_frow[4 * rg + li];
-#pragma unroll
-      for (int hg = 0; hg < NHG; ++hg) {
-        const unsigned o = (hg == NHG - 1 && (NHE % 4) != 0) ? lrow_last + fro : lrow + fro;
-        ea[hg] = buf_ld(ub, o, (unsigned)hg * hstep);
-      }
-    };
-    load_rows(0);
     for (int rg = 0; rg < nrg; ++rg) {
       double e[NHG];
 #pragma unroll
@@ -502,8 +521,8 @@ __global__ void __launch_bounds__(64) MD4_ATTR md4s_kernel(MdArgs A) {
 
   for (long long t = v0; t < ntiles; t += G) {
     const long long rb = t / nct, ct = t - rb * nct;
-    const long long ir0 = rb * M4_NB, jc0 = ct * M4S_NK;
-    const int nb = (int)((A.nrows - ir0) < M4_NB ? (A.nrows - ir0) : M4_NB);
+    const long long ir0 = rb * nbr, jc0 = ct * M4S_NK;
+    const int nb = (int)((A.nrows - ir0) < nbr ? (A.nrows - ir0) : nbr);
     const int nk = (int)((A.ncols - jc0) < M4S_NK ? (A.ncols - jc0) : M4S_NK);
     // this lane's column: ket pair jc0 + col, its single orbital pair
     const bool cval = col < nk;
@@ -524,9 +543,11 @@ __global__ void __launch_bounds__(64) MD4_ATTR md4s_kernel(MdArgs A) {
       for (int b = 0; b < nb; ++b) {
         const QRec RB = A.rrec[ir0 + b];
         fill_rows(RB);
+        s2_open(RB, A.et + RB.et);
+        load_rows(0);
         build_r(16 * b);
         stage1(etv, false);
-        stage2(RB, A.et + RB.et, outc, cval, false);
+        stage2(RB, outc, cval, false);
       }
     } else {
       // contracted pairs: per bra pair and bra primitive pair, X summed over
@@ -539,6 +560,8 @@ __global__ void __launch_bounds__(64) MD4_ATTR md4s_kernel(MdArgs A) {
         fill_rows(RB);
         for (int bp = 0; bp < RB.nprim; ++bp) {
           const PrimRec PB = bp == 0 ? first_prim(RB) : A.prim[RB.prim0 + bp];
+          s2_open(RB, A.et + PB.et);
+          load_rows(0);
           for (int kp = 0; kp < maxnp; ++kp) {
             clear_boys();
             if (lane < nk) {
@@ -556,7 +579,7 @@ __global__ void __launch_bounds__(64) MD4_ATTR md4s_kernel(MdArgs A) {
             }
             stage1(etv, kp > 0);
           }
-          stage2(RB, A.et + PB.et, outc, cval, bp > 0);
+          stage2(RB, outc, cval, bp > 0);
         }
       }
     }

@@ -85,15 +85,18 @@ THICK = [(3, 3), (3, 2), (2, 3), (2, 2), (3, 1)]
 
 @pytest.mark.parametrize("rows,cols", [(ALL_L, ALL_L), (THICK, THIN), (THIN, THICK)],
                          ids=["mixed", "thin-kets", "thin-bras"])
-@pytest.mark.parametrize("lane,trans,kt,md4", [(1, 1, 32, 1), (0, 1, 32, 1), (0, 0, 32, 1), (0, 1, 32, 0),
-                                               (0, 0, 32, 0), (0, 1, 1, 0)],
-                         ids=["default", "md4-only", "md4-no-transpose", "md_kernel", "md_kernel-no-transpose",
-                              "md_kernel-one-ket-tiles"])
-def test_pair_block_launch_shapes(setup, rows, cols, lane, trans, kt, md4):
+@pytest.mark.parametrize("lane,trans,kt,md4,md4_tiles", [(1, 1, 32, 1, 2048), (1, 1, 32, 1, 0), (1, 1, 32, 1, 6),
+                                                         (0, 1, 32, 1, 2048), (0, 0, 32, 1, 2048),
+                                                         (0, 1, 32, 0, 2048), (0, 0, 32, 0, 2048),
+                                                         (0, 1, 1, 0, 2048)],
+                         ids=["default", "md4-four-rows", "md4-two-rows", "md4-only", "md4-no-transpose",
+                              "md_kernel", "md_kernel-no-transpose", "md_kernel-one-ket-tiles"])
+def test_pair_block_launch_shapes(setup, rows, cols, lane, trans, kt, md4, md4_tiles):
     """The product launches with every kernel choice: the 4x4x4 kernel
-    (column tiles of four slots, per-class orientation), the 16x16x4 kernel
-    (ket stacking across pairs of one tile, transposed orientation,
-    single-ket tiles), lane kernels for the small classes."""
+    (column tiles of four slots, per-class orientation, row blocks of one,
+    two or four bra pairs), the 16x16x4 kernel (ket stacking across pairs of
+    one tile, transposed orientation, single-ket tiles), lane kernels for the
+    small classes."""
     b, o, ctx = setup
     rng = np.random.default_rng(7)
     rp, cp = _pairs(b, rng, rows, 37), _pairs(b, rng, cols, 41)
@@ -103,6 +106,7 @@ def test_pair_block_launch_shapes(setup, rows, cols, lane, trans, kt, md4):
         ctx.set_option("eri_lane", lane)
         ctx.set_option("eri_trans", trans)
         ctx.set_option("eri_md4", md4)
+        ctx.set_option("eri_md4_tiles", md4_tiles)
         V = ctx.eri_pair_block(rp, cp)
     finally:
         ctx.set_option("eri_min_tiles", 8192)
@@ -110,6 +114,7 @@ def test_pair_block_launch_shapes(setup, rows, cols, lane, trans, kt, md4):
         ctx.set_option("eri_lane", 1)
         ctx.set_option("eri_trans", 1)
         ctx.set_option("eri_md4", 1)
+        ctx.set_option("eri_md4_tiles", 2048)
     nang = b.shell_nang
     r0 = 0
     worst = 0.0

@@ -111,12 +111,15 @@ def _pair_block_vs_oracle(b, o, V, rp, cp, tol):
 
 
 @pytest.mark.parametrize("contracted", [False, True], ids=["plain", "contracted"])
-@pytest.mark.parametrize("thin_emin", [0, 2, 7], ids=["thin-all", "thin-default", "thin-off"])
-def test_thin_ket_pair_blocks(gpu_ctx, oracle_mod, contracted, thin_emin):
+@pytest.mark.parametrize("thin_emin,md4_tiles", [(0, 0), (0, 40), (0, 2048), (2, 2048), (7, 0)],
+                         ids=["thin-all-four-rows", "thin-all-two-rows", "thin-all-one-row", "thin-default",
+                              "thin-off"])
+def test_thin_ket_pair_blocks(gpu_ctx, oracle_mod, contracted, thin_emin, md4_tiles):
     """Products with an s-s side: the thin-ket 4x4x4 kernel (sixteen s-s kets
-    per tile, one quartet per column) against the oracle, with the s-s pairs
-    as the caller's columns and as its rows (transposed launch), bra orders
-    0 .. 6, more s-s pairs than one tile holds."""
+    per tile, one quartet per column, row blocks of four, two or one bra
+    pairs) against the oracle, with the s-s pairs as the caller's columns and
+    as its rows (transposed launch), bra orders 0 .. 6, more s-s pairs than
+    one tile holds."""
     b = systems.water_system(2, 6.0, "gh", seed=13, contracted=contracted)
     o = oracle_mod.Oracle(b)
     gpu_ctx.set_basis(b)
@@ -133,11 +136,14 @@ def test_thin_ket_pair_blocks(gpu_ctx, oracle_mod, contracted, thin_emin):
     try:
         gpu_ctx.set_option("eri_min_tiles", 1)
         gpu_ctx.set_option("eri_thin_emin", thin_emin)
+        gpu_ctx.set_option("eri_md4_tiles", md4_tiles)
         V1 = gpu_ctx.eri_pair_block(thick, ss)
         V2 = gpu_ctx.eri_pair_block(ss, thick)
     finally:
         gpu_ctx.set_option("eri_min_tiles", 8192)
         gpu_ctx.set_option("eri_thin_emin", 2)
+        gpu_ctx.set_option("eri_md4_tiles", 2048)
     w1 = _pair_block_vs_oracle(b, o, V1, thick, ss, tol)
     w2 = _pair_block_vs_oracle(b, o, V2, ss, thick, tol)
-    print("thin_emin %d: max |gpu - oracle| = %.2e (s-s columns), %.2e (s-s rows)" % (thin_emin, w1, w2))
+    print("thin_emin %d, md4_tiles %d: max |gpu - oracle| = %.2e (s-s columns), %.2e (s-s rows)"
+          % (thin_emin, md4_tiles, w1, w2))
