@@ -365,6 +365,17 @@ constexpr int POTF_NB = 64;
 
 __global__ void k_set_int(int *p, int v) { *p = v; }
 
+// rows g msd + dim[g] .. (g + 1) msd - 1 of every column of W (ldw), for the
+// ng subsets of a Vabrs group: the padding below each subset's dimension,
+// which no ERI block writes
+__global__ void k_zero_pad_rows(double *__restrict__ W, long long ldw, long long msd,
+                                const long long *__restrict__ dim, long long ncol) {
+  const long long g = blockIdx.y;
+  const long long r0 = g * msd + dim[g], r1 = (g + 1) * msd;
+  for (long long c = blockIdx.x; c < ncol; c += gridDim.x)
+    for (long long r = r0 + threadIdx.x; r < r1; r += blockDim.x) W[r + ldw * c] = 0.0;
+}
+
 // Cholesky factor L and its inverse X of the diagonal block at k0 (b <= POTF_NB
 // rows, lower triangle of V read); writes L (zero above the diagonal) over the
 // block of V and X into the block of X.  Does nothing once an earlier block failed.
@@ -1144,6 +1155,7 @@ void chol_build(Context &C, double tau, double kappa, long long max_block_dim, l
     per_group = std::min(per_group, nsub);
     DBuf<double> &dW = chol_work(C).Wv;
     dW.alloc(slab * per_group);
+    DBuf<long long> gpad;   // subset dimensions of the group
     double t_vab = 0, t_gemm = 0;
     for (long long X0 = 1; X0 <= nsub; X0 += per_group) {
       long long X1 = std::min(nsub, X0 + per_group - 1);
@@ -1151,7 +1163,16 @@ void chol_build(Context &C, double tau, double kappa, long long max_block_dim, l
       // Vabrs of the group's subsets stacked into one (ng msd) x NVecs matrix:
       // row (X - X0) msd + subset position, column = pivot
       const long long ldw = msd * ng;
-      GI_HIP(hipMemsetAsync(dW.p, 0, sizeof(double) * slab * ng, C.stream));
+      // the ERI blocks below cover rows 0 .. sdim[X] - 1 of each subset and
+      // every pivot column; only the padding rows up to msd need zeros
+      {
+        std::vector<long long> gdim(ng);
+        for (long long X = X0; X <= X1; ++X) gdim[X - X0] = sdim[X];
+        gpad.upload(gdim, C.stream);
+        hipLaunchKernelGGL(k_zero_pad_rows, dim3((unsigned)std::min<long long>(nvec, 1024), (unsigned)ng), dim3(64), 0,
+                           C.stream, dW.p, ldw, msd, gpad.p, nvec);
+        GI_HIP(hipGetLastError());
+      }
       std::vector<PairDesc> cd(npsp);
       std::vector<int> cols(npsp);
       for (long long k = 0; k < npsp; ++k) {

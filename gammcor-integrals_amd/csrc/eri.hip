@@ -294,15 +294,26 @@ static long long build_col_tiles(const std::vector<int> &ncol, int qmax, std::ve
 void eri_run(Context &C, const EriRequest &R) {
   init_md_table();
   std::vector<PairDesc> rd = *R.rdesc, cd = *R.cdesc;
-  // ---- pair tables: look up the basis-level cache, build the missing ones
+  // ---- pair tables: look up the basis-level cache (a dense shell-pair
+  // index, or a hash map for very large bases), build the missing ones
   std::vector<EtJob> jobs;
   const size_t nhdr0 = C.md_hdr_host.size();
+  const bool dense = !C.md_pt_dense.empty();
   auto pair_entry = [&](PairDesc &d) {
+    if (d.s0 < 0 || d.s0 >= C.nshells || d.s1 < 0 || d.s1 >= C.nshells) throw Error{8, "shell index out of range"};
     const long long key = (long long)d.s0 * C.nshells + d.s1;
-    auto it = C.md_pt.find(key);
-    if (it != C.md_pt.end()) {
-      d.pt = it->second;
-      return;
+    if (dense) {
+      const int pt = C.md_pt_dense[(size_t)key];
+      if (pt >= 0) {
+        d.pt = pt;
+        return;
+      }
+    } else {
+      auto it = C.md_pt.find(key);
+      if (it != C.md_pt.end()) {
+        d.pt = it->second;
+        return;
+      }
     }
     const int l0 = C.shell_l[d.s0], l1 = C.shell_l[d.s1];
     if (l0 + l1 > 2 * MAXL) {
@@ -319,19 +330,29 @@ void eri_run(Context &C, const EriRequest &R) {
         C.md_et_used += (long long)nherm(l0 + l1) * n0 * n1;
       }
     d.pt = (int)C.md_hdr_host.size();
-    C.md_pt[key] = d.pt;
+    if (dense)
+      C.md_pt_dense[(size_t)key] = d.pt;
+    else
+      C.md_pt[key] = d.pt;
     C.md_hdr_host.push_back(h);
   };
   for (int r : *R.rows) pair_entry(rd[r]);
   for (int c : *R.cols) pair_entry(cd[c]);
-  for (int r : *R.rows) {
-    const PairDesc &d = rd[r];
-    if (pair_rows_host(d, C.shell_nang[d.s0], C.shell_nang[d.s1]) <= 0) throw Error{8, "empty bra pair"};
-  }
-  for (int c : *R.cols) {
-    const PairDesc &d = cd[c];
-    if (pair_rows_host(d, C.shell_nang[d.s0], C.shell_nang[d.s1]) <= 0) throw Error{8, "empty ket pair"};
-  }
+  // rows (orbital pairs), primitive pairs and E = l0 + l1 of every pair used
+  std::vector<int> rnr(rd.size(), 0), rnp(rd.size(), 0), rE(rd.size(), 0);
+  std::vector<int> cnr(cd.size(), 0), cnp(cd.size(), 0), cE(cd.size(), 0);
+  auto describe = [&](const std::vector<PairDesc> &d, const std::vector<int> &use, std::vector<int> &nr,
+                      std::vector<int> &np, std::vector<int> &E, const char *what) {
+    for (int k : use) {
+      const PairDesc &x = d[k];
+      nr[k] = pair_rows_host(x, C.shell_nang[x.s0], C.shell_nang[x.s1]);
+      if (nr[k] <= 0) throw Error{8, std::string("empty ") + what + " pair"};
+      np[k] = C.shell_np[x.s0] * C.shell_np[x.s1];
+      E[k] = C.shell_l[x.s0] + C.shell_l[x.s1];
+    }
+  };
+  describe(rd, *R.rows, rnr, rnp, rE, "bra");
+  describe(cd, *R.cols, cnr, cnp, cE, "ket");
   if (R.rows->empty() || R.cols->empty()) return;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (C.time_eri) {
@@ -347,43 +368,82 @@ void eri_run(Context &C, const EriRequest &R) {
     C.md_et.grow_keep((size_t)C.md_et_used, C.stream);
   }
 
-  // ---- bins by E (rows) and F (columns)
-  auto E_of = [&](const PairDesc &d) { return C.shell_l[d.s0] + C.shell_l[d.s1]; };
+  // ---- segments: lists of pairs packed into consecutive records.  Products
+  // bin rows by E and columns by F, one segment per non-empty bin, shared by
+  // every (E, F) launch of the bin; zipped lists give each (E, F) class its
+  // own row and column segments (pairs matched by position).
+  constexpr int NB = 2 * MAXL + 1;
+  struct Seg {
+    int side;               // 0: rows (rd), 1: columns (cd)
+    std::vector<int> idx;   // descriptor indices
+    size_t off = 0;         // first record
+    // aggregates over the segment: primitive pairs, rows, rows x primitive
+    // pairs, primitive pairs x ceil(rows / 4); every pair uncontracted / one row
+    double s_np = 0, s_nr = 0, s_nrnp = 0, s_r4np = 0;
+    bool plain = true, nr1 = true;
+  };
+  std::vector<Seg> segs;
   struct Launch {
     int E, F;
-    std::vector<int> rows, cols;
+    int rs, cs;   // row and column segments
   };
   std::vector<Launch> launches;
   if (R.mode == LAUNCH_PRODUCT) {
-    std::map<int, std::vector<int>> rb, cb;
-    for (int r : *R.rows) rb[E_of(rd[r])].push_back(r);
-    for (int c : *R.cols) cb[E_of(cd[c])].push_back(c);
-    for (auto &x : rb)
-      for (auto &y : cb) launches.push_back(Launch{x.first, y.first, x.second, y.second});
+    int rbin[NB], cbin[NB];
+    for (int e = 0; e < NB; ++e) rbin[e] = cbin[e] = -1;
+    for (int r : *R.rows) {
+      int &b = rbin[rE[r]];
+      if (b < 0) {
+        b = (int)segs.size();
+        segs.push_back(Seg{0, {}});
+      }
+      segs[b].idx.push_back(r);
+    }
+    for (int c : *R.cols) {
+      int &b = cbin[cE[c]];
+      if (b < 0) {
+        b = (int)segs.size();
+        segs.push_back(Seg{1, {}});
+      }
+      segs[b].idx.push_back(c);
+    }
+    for (int e = 0; e < NB; ++e)
+      for (int f = 0; f < NB; ++f)
+        if (rbin[e] >= 0 && cbin[f] >= 0) launches.push_back(Launch{e, f, rbin[e], cbin[f]});
   } else {
-    std::map<std::pair<int, int>, Launch> zb;
+    int zb[NB][NB];
+    for (int e = 0; e < NB; ++e)
+      for (int f = 0; f < NB; ++f) zb[e][f] = -1;
     for (size_t i = 0; i < R.rows->size(); ++i) {
       const int r = (*R.rows)[i], c = (*R.cols)[i];
-      const int E = E_of(rd[r]), F = E_of(cd[c]);
-      auto &L = zb[{E, F}];
-      L.E = E;
-      L.F = F;
-      L.rows.push_back(r);
-      L.cols.push_back(c);
+      int &z = zb[rE[r]][cE[c]];
+      if (z < 0) {
+        z = (int)launches.size();
+        launches.push_back(Launch{rE[r], cE[c], (int)segs.size(), (int)segs.size() + 1});
+        segs.push_back(Seg{0, {}});
+        segs.push_back(Seg{1, {}});
+      }
+      segs[launches[z].rs].idx.push_back(r);
+      segs[launches[z].cs].idx.push_back(c);
     }
-    for (auto &kv : zb) launches.push_back(std::move(kv.second));
   }
   std::vector<int> idx;
-  std::vector<size_t> roff(launches.size()), coff(launches.size());
-  for (size_t i = 0; i < launches.size(); ++i) {
-    roff[i] = idx.size();
-    idx.insert(idx.end(), launches[i].rows.begin(), launches[i].rows.end());
-    coff[i] = idx.size();
-    idx.insert(idx.end(), launches[i].cols.begin(), launches[i].cols.end());
+  std::vector<unsigned char> side;
+  for (Seg &g : segs) {
+    const std::vector<int> &nr = g.side ? cnr : rnr, &np = g.side ? cnp : rnp;
+    for (int k : g.idx) {
+      const double n = nr[k], p = np[k];
+      g.s_np += p;
+      g.s_nr += n;
+      g.s_nrnp += n * p;
+      g.s_r4np += p * (double)((nr[k] + 3) / 4);
+      g.plain = g.plain && np[k] == 1;
+      g.nr1 = g.nr1 && nr[k] == 1;
+    }
+    g.off = idx.size();
+    idx.insert(idx.end(), g.idx.begin(), g.idx.end());
+    side.insert(side.end(), g.idx.size(), (unsigned char)g.side);
   }
-  std::vector<unsigned char> side(idx.size(), 0);
-  for (size_t i = 0; i < launches.size(); ++i)
-    std::fill(side.begin() + coff[i], side.begin() + coff[i] + launches[i].cols.size(), (unsigned char)1);
   C.md_rec.alloc(idx.size());
 
   static const bool stats = getenv("GIGPU_ERI_STATS") != nullptr;
@@ -403,15 +463,32 @@ void eri_run(Context &C, const EriRequest &R) {
   };
   std::vector<Prepared> prep;
   std::vector<ColTile> ctiles;
-  auto nrow_of = [&](const PairDesc &d) {
-    return (double)pair_rows_host(d, C.shell_nang[d.s0], C.shell_nang[d.s1]);
+  // column tiles of a segment as the 4x4x4 kernel's ket side, per qmax
+  struct Tiles {
+    std::vector<ColTile> t;
+    long long nrt = 0;
   };
-  auto nprim_of = [&](const PairDesc &d) { return (double)C.shell_np[d.s0] * C.shell_np[d.s1]; };
+  std::map<std::pair<int, int>, Tiles> tile_cache;
+  auto seg_tiles = [&](int sg, int qmax) -> const Tiles & {
+    auto key = std::make_pair(sg, qmax);
+    auto it = tile_cache.find(key);
+    if (it != tile_cache.end()) return it->second;
+    const Seg &g = segs[sg];
+    const std::vector<int> &nr = g.side ? cnr : rnr;
+    std::vector<int> nc;
+    nc.reserve(g.idx.size());
+    for (int k : g.idx) nc.push_back(nr[k]);
+    Tiles T;
+    T.nrt = build_col_tiles(nc, qmax, T.t);
+    return tile_cache.emplace(key, std::move(T)).first->second;
+  };
   for (size_t i = 0; i < launches.size(); ++i) {
-    auto &L = launches[i];
-    long long nrows = (long long)L.rows.size(), ncols = (long long)L.cols.size();
+    const Launch &L = launches[i];
+    const Seg &SR = segs[L.rs], &SC = segs[L.cs];
+    long long nrows = (long long)SR.idx.size(), ncols = (long long)SC.idx.size();
     if (nrows == 0 || ncols == 0) continue;
     const long long nq = (R.mode == LAUNCH_PRODUCT) ? nrows * ncols : nrows;
+    const bool plain = SR.plain && SC.plain;
     // lane-per-quartet kernel for small uncontracted classes
     // the lane kernels want the thin side (lower Hermite order) as their bra:
     // a product with E > F runs transposed on the (F, E) kernel
@@ -433,8 +510,7 @@ void eri_run(Context &C, const EriRequest &R) {
       const bool lane_wins = (e <= 1 && f <= 3) || (f == 0 && e <= 3) || (e == 2 && f == 0);
       if (!lane_wins) lane_path = false, trans = false;
     }
-    for (size_t k = 0; lane_path && k < L.rows.size(); ++k) lane_path = nprim_of(rd[L.rows[k]]) == 1;
-    for (size_t k = 0; lane_path && k < L.cols.size(); ++k) lane_path = nprim_of(cd[L.cols[k]]) == 1;
+    lane_path = lane_path && plain;
     if (!lane_path) trans = false;
     // s-s side: the thin-ket 4x4x4 kernel with the s-s pairs as its ket (one
     // column each, sixteen quartets per instruction)
@@ -442,8 +518,7 @@ void eri_run(Context &C, const EriRequest &R) {
     if (!generic && R.mode == LAUNCH_PRODUCT && C.eri_md4 && std::min(L.E, L.F) == 0 &&
         std::max(L.E, L.F) >= C.eri_thin_emin) {
       const bool tt = L.F != 0;   // E = 0 < F: the caller's rows are the s-s side
-      thin = !tt || C.eri_trans;
-      for (int c : (tt ? L.rows : L.cols)) thin = thin && nrow_of((tt ? rd : cd)[c]) == 1.0;
+      thin = (!tt || C.eri_trans) && (tt ? SR.nr1 : SC.nr1);
       if (thin) {
         lane_path = false;
         trans = tt;
@@ -451,82 +526,50 @@ void eri_run(Context &C, const EriRequest &R) {
     }
     // MFMA kernel orientation: its bra side (stage-2 rows, 16-row tiles) is
     // the side with more rows per pair; the thin side is stacked in columns
-    if (!thin && !lane_path && !generic && R.mode == LAUNCH_PRODUCT && C.eri_trans) {
-      double sr = 0, sc = 0;
-      for (int r : L.rows) sr += nrow_of(rd[r]);
-      for (int c : L.cols) sc += nrow_of(cd[c]);
-      trans = sc / (double)ncols > sr / (double)nrows;
-    }
+    if (!thin && !lane_path && !generic && R.mode == LAUNCH_PRODUCT && C.eri_trans)
+      trans = SC.s_nr / (double)ncols > SR.s_nr / (double)nrows;
     // 4x4x4 kernel (products): orientation and column tiles from its padded
     // instruction count -- per tile and bra primitive pair, NHG (KSF ket
     // primitives + ceil(nr / 4)) instructions, plus the R tables (about one
     // instruction's time per 40 recursion entries)
-    bool use4 = !thin && !lane_path && !generic && R.mode == LAUNCH_PRODUCT && C.eri_md4;
-    std::vector<ColTile> t4[2];
+    const bool use4 = !thin && !lane_path && !generic && R.mode == LAUNCH_PRODUCT && C.eri_md4;
+    const Tiles *t4 = nullptr;
     if (use4) {
       double best = 0;
-      int bo = 0;
       for (int o = 0; o < 2; ++o) {
         if (o == 1 && !C.eri_trans) break;
         const int e = o ? L.F : L.E, f = o ? L.E : L.F;
-        const std::vector<PairDesc> &ord = o ? cd : rd, &ocd = o ? rd : cd;
-        const std::vector<int> &orows = o ? L.cols : L.rows, &ocols = o ? L.rows : L.cols;
-        std::vector<int> nc;
-        nc.reserve(ocols.size());
-        double np_c = 0;
-        for (int c : ocols) {
-          nc.push_back((int)nrow_of(ocd[c]));
-          np_c += nprim_of(ocd[c]);
-        }
-        np_c /= (double)std::max<size_t>(1, ocols.size());
-        const long long nrt = build_col_tiles(nc, m4_qmax(e + f), t4[o]);
+        const Seg &bra = o ? SC : SR, &ket = o ? SR : SC;
+        const Tiles &T = seg_tiles(o ? L.rs : L.cs, m4_qmax(e + f));
+        const double np_c = ket.s_np / (double)std::max<size_t>(1, ket.idx.size());
         const double nhg = (nherm(e) + 3) / 4, ksf = (nherm(f) + 3) / 4;
         double rentries = 0;
         for (int m = 0; m <= e + f; ++m) rentries += nherm(m);
-        double sum_rows = 0, sum_np = 0;
-        for (int r : orows) {
-          const double p = nprim_of(ord[r]);
-          sum_rows += p * std::ceil(nrow_of(ord[r]) / 4.0);
-          sum_np += p;
-        }
-        const double cost = (double)t4[o].size() * nhg * (ksf * np_c * sum_np + sum_rows) +
-                            (double)nrt * np_c * sum_np * rentries / 40.0;
+        const double cost = (double)T.t.size() * nhg * (ksf * np_c * bra.s_np + bra.s_r4np) +
+                            (double)T.nrt * np_c * bra.s_np * rentries / 40.0;
         if (o == 0 || cost < best) {
           best = cost;
-          bo = o;
+          trans = o == 1;
+          t4 = &T;
         }
       }
-      trans = bo == 1;
     }
     const int KE = trans ? L.F : L.E, KF = trans ? L.E : L.F;   // kernel bra / ket orders
-    const std::vector<PairDesc> &krd = trans ? cd : rd, &kcd = trans ? rd : cd;
-    const std::vector<int> &krows = trans ? L.cols : L.rows, &kcols = trans ? L.rows : L.cols;
+    const Seg &KR = trans ? SC : SR, &KC = trans ? SR : SC;
     if (trans) std::swap(nrows, ncols);
     // algorithmic operation count in the kernel's own roles (useful rows/columns only)
     const double NHE = nherm(KE), NHF = nherm(KF), rt = rtable_flops(L.E + L.F);
     double fl = 0;
     if (R.mode == LAUNCH_PRODUCT) {
-      double sr_np = 0, sr_nr = 0, sr_nrnp = 0, sc_np = 0, sc_nc = 0, sc_ncnp = 0;
-      for (int r : krows) {
-        const double n = nrow_of(krd[r]), p = nprim_of(krd[r]);
-        sr_np += p;
-        sr_nr += n;
-        sr_nrnp += n * p;
-      }
-      for (int c : kcols) {
-        const double n = nrow_of(kcd[c]), p = nprim_of(kcd[c]);
-        sc_np += p;
-        sc_nc += n;
-        sc_ncnp += n * p;
-      }
       if (lane_path)   // 2 nr NHF (NHE + nc) per quartet
-        fl = 2.0 * NHF * (NHE * sr_nr * (double)ncols + sr_nr * sc_nc) + rt * (double)nrows * (double)ncols;
+        fl = 2.0 * NHF * (NHE * KR.s_nr * (double)ncols + KR.s_nr * KC.s_nr) + rt * (double)nrows * (double)ncols;
       else             // stage 1 2 NHE NHF nc per primitive quartet, stage 2 2 nr NHE nc per bra primitive pair
-        fl = 2.0 * NHE * NHF * sr_np * sc_ncnp + 2.0 * NHE * sr_nrnp * sc_nc + rt * sr_np * sc_np;
+        fl = 2.0 * NHE * NHF * KR.s_np * KC.s_nrnp + 2.0 * NHE * KR.s_nrnp * KC.s_nr + rt * KR.s_np * KC.s_np;
     } else {
-      for (size_t k = 0; k < krows.size(); ++k) {
-        const double nr = nrow_of(krd[krows[k]]), pr = nprim_of(krd[krows[k]]);
-        const double nc = nrow_of(kcd[kcols[k]]), pc = nprim_of(kcd[kcols[k]]);
+      const std::vector<int> &kr = KR.idx, &kc = KC.idx;
+      for (size_t k = 0; k < kr.size(); ++k) {
+        const double nr = rnr[kr[k]], pr = rnp[kr[k]];
+        const double nc = cnr[kc[k]], pc = cnp[kc[k]];
         if (lane_path)
           fl += 2.0 * nr * NHF * (NHE + (R.mode == LAUNCH_DIAG ? 1.0 : nc)) + rt;
         else
@@ -537,8 +580,8 @@ void eri_run(Context &C, const EriRequest &R) {
     MdArgs a{};
     a.prim = C.md_prim.p;
     a.et = C.md_et.p;
-    a.rrec = C.md_rec.p + (trans ? coff[i] : roff[i]);
-    a.crec = C.md_rec.p + (trans ? roff[i] : coff[i]);
+    a.rrec = C.md_rec.p + KR.off;
+    a.crec = C.md_rec.p + KC.off;
     a.nrows = nrows;
     a.ncols = ncols;
     a.mode = R.mode;
@@ -565,10 +608,9 @@ void eri_run(Context &C, const EriRequest &R) {
       grid = std::min<long long>(a.ntiles, 8192);
       if (grid >= 8) grid = (grid + 7) / 8 * 8;
     } else if (use4) {
-      std::vector<ColTile> &tl = t4[trans ? 1 : 0];
       ct0 = (long long)ctiles.size();
-      nct = (long long)tl.size();
-      ctiles.insert(ctiles.end(), tl.begin(), tl.end());
+      nct = (long long)t4->t.size();
+      ctiles.insert(ctiles.end(), t4->t.begin(), t4->t.end());
       a.kt = row_block(nct);
       a.ntiles = (nrows + a.kt - 1) / a.kt * nct;   // row blocks x column tiles
       grid = std::min<long long>(a.ntiles, 8192);
@@ -582,9 +624,7 @@ void eri_run(Context &C, const EriRequest &R) {
       if (R.mode == LAUNCH_PRODUCT) {
         // enough ket pairs per tile to fill 32 stacked columns, while keeping
         // at least eri_min_tiles tiles when the launch is large enough
-        double sc = 0;
-        for (int c : kcols) sc += nrow_of(kcd[c]);
-        long long kt = (long long)std::ceil(32.0 / std::max(1.0, sc / (double)ncols));
+        long long kt = (long long)std::ceil(32.0 / std::max(1.0, KC.s_nr / (double)ncols));
         kt = std::max<long long>(1, std::min<long long>(kt, C.eri_max_kt));
         while (kt > 1 && nrows * ((ncols + kt - 1) / kt) < C.eri_min_tiles) kt = (kt + 1) / 2;
         a.kt = (int)kt;
@@ -595,9 +635,6 @@ void eri_run(Context &C, const EriRequest &R) {
       grid = std::min<long long>(a.ntiles, 4096);
       if (grid >= 8) grid = (grid + 7) / 8 * 8;
     }
-    bool plain = true;
-    for (size_t k = 0; plain && k < L.rows.size(); ++k) plain = nprim_of(rd[L.rows[k]]) == 1;
-    for (size_t k = 0; plain && k < L.cols.size(); ++k) plain = nprim_of(cd[L.cols[k]]) == 1;
     prep.push_back(Prepared{a, grid, thin ? 4 : (use4 ? 3 : (generic ? 2 : (lane_path ? 1 : 0))), KE, KF, L.E, L.F,
                             nq, fl, ct0, nct, plain});
     nq_total += nq;

@@ -105,6 +105,7 @@ struct Context {
   // ERI pair tables, cached per basis: shell pair (s0, s1) -> entry of
   // md_hdr_host; PrimRec and ET blocks on the device grow as pairs are added
   std::unordered_map<long long, int> md_pt;
+  std::vector<int> md_pt_dense;   // s0 * nshells + s1 -> entry or -1 (bases up to 4096 shells)
   std::vector<PairHdr> md_hdr_host;
   long long md_et_used = 0, md_prim_used = 0;
   DBuf<PairHdr> md_hdr;
@@ -122,6 +123,10 @@ struct Context {
   unsigned char *stage_host(size_t n);
   void reset_pair_cache() {
     md_pt.clear();
+    if (nshells > 0 && nshells <= 4096)
+      md_pt_dense.assign((size_t)nshells * (size_t)nshells, -1);
+    else
+      md_pt_dense.clear();
     md_hdr_host.clear();
     md_et_used = md_prim_used = 0;
   }
