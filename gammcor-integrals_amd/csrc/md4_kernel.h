@@ -224,7 +224,10 @@ __global__ void __launch_bounds__(64) MD4_ATTR md4_kernel(MdArgs A, const ColTil
     const int fc = cval ? pair_full(CD, A.sel, c) : 0;
     const int ldk = CD.n0 * CD.n1;
 
-    // stage-1 B operand of ket primitive kp: (-1)^{|k|} ET_ket(k, c)
+    // stage-1 B operand of ket primitive kp: (-1)^{|k|} ET_ket(k, c).  Every
+    // lane loads (index 0 where k or c is outside the table) and scales by
+    // +-1 or 0 afterwards: the loads are all in flight together instead of
+    // each waiting behind a branch
     double eb[KSF];
     auto load_eb = [&](int kp) {
       const bool kval = cval && kp < CD.nprim;
@@ -235,8 +238,12 @@ __global__ void __launch_bounds__(64) MD4_ATTR md4_kernel(MdArgs A, const ColTil
       for (int kg = 0; kg < KSF; ++kg) {
         const int k = 4 * kg + lq;
         const bool ok = kval && k < NHF;
-        const double v = etk[ok ? (long long)k * ldk + fc : 0];
-        eb[kg] = ok ? (((ksgn >> kg) & 1u) ? -v : v) : 0.0;
+        eb[kg] = etk[ok ? k * ldk + fc : 0];
+      }
+#pragma unroll
+      for (int kg = 0; kg < KSF; ++kg) {
+        const bool ok = kval && 4 * kg + lq < NHF;
+        eb[kg] *= ok ? (((ksgn >> kg) & 1u) ? -1.0 : 1.0) : 0.0;
       }
     };
     // stage 1: X(h, c) += sum_k R_{h+k} B(k, c) from the dense table of this
