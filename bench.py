@@ -27,9 +27,11 @@ import time
 import numpy as np
 
 # Hardware queues per process (HIP's default is 4): the batched form runs one
-# stream per molecule, and with 4 queues at most four of them run
-# concurrently.  Read by the HIP runtime at initialisation, before any HIP call.
-os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
+# stream per molecule, and with 4 queues at most four kernels run
+# concurrently (measured: 0.60 s with 4, 0.55 s with 16).  Read by the HIP
+# runtime at initialisation, before any HIP call; set here even when the
+# environment already holds a value.
+os.environ["GPU_MAX_HW_QUEUES"] = "16"
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "gammcor-integrals_amd"))
@@ -282,7 +284,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mo", action="store_true")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-form comparison")
-    ap.add_argument("--batch-workers", type=int, default=12)
+    ap.add_argument("--batch-workers", type=int, default=16)
     ap.add_argument("--no-autotune", action="store_true", help="skip autotuning the batched form")
     ap.add_argument("--no-gradient", action="store_true", help="skip the differentiation-form timing")
     ap.add_argument("--no-precision", action="store_true", help="skip the split-BF16 precision-form comparison")

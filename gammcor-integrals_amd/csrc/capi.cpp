@@ -220,6 +220,7 @@ int gi_set_option(gi_context *cp, const char *name, int64_t value) {
     else if (n == "eri_md4") C.eri_md4 = value != 0;
     else if (n == "eri_thin_emin") C.eri_thin_emin = std::max<int64_t>(0, value);
     else if (n == "eri_md4_tiles") C.eri_md4_tiles = std::max<int64_t>(0, value);
+    else if (n == "micro_persist_max") C.micro_persist_max = std::max<int64_t>(0, value);
     else if (n == "eri_lane_emax") C.eri_lane_emax = std::max<int64_t>(0, value);
     else if (n == "vec_blocks") C.vec_blocks = std::max<int64_t>(1, value);
     else if (n == "eri_streams")
@@ -246,16 +247,20 @@ int gi_chol_build_batch(int64_t n, gi_context *const *ctxs, double tau, double k
           gi::Context &C = ctx_of(ctxs[i]);
           // the molecules of the batch already run concurrently (one stream
           // each); forking every context's ERI launches over side streams as
-          // well would oversubscribe the hardware queues
+          // well would oversubscribe the hardware queues.  Its waits sleep
+          // instead of spinning: every worker thread waits on the GPU at once
           const long long streams = C.eri_streams;
           C.eri_streams = 1;
+          C.blocking_waits = true;
           try {
             gi::chol_build(C, tau, kappa, max_block_dim, vec_block, flags);
           } catch (...) {
             C.eri_streams = streams;
+            C.blocking_waits = false;
             throw;
           }
           C.eri_streams = streams;
+          C.blocking_waits = false;
           status[i] = 0;
         } catch (const gi::Error &e) {
           status[i] = e.code ? e.code : 1;

@@ -228,24 +228,18 @@ __device__ void wg_argmax_1024(double &k, double &v, int &i) {
 }
 
 // One panel of micro iterations (chol2_Lj + D update + BatchMax,
-// TwoStepCholesky_Step1.f90:1027-1064) on the batch rows, nb <= MICRO_ROWS:
+// TwoStepCholesky_Step1.f90:1027-1064) on the batch rows, nb <= MICRO_ROWS,
+// by a workgroup of MICRO_ROWS lanes (lane i = batch row i):
 //   L(i, j) = (A(i, e) - sum_{p0 <= q < j} L(i, q) L(e, q)) / sqrt(D(e)),
 //   D(i) -= L(i, j)^2,  e <- first maximum of D (quantized keys).
-__global__ void __launch_bounds__(MICRO_ROWS) k_micro_panel(MicroState *st, const double *__restrict__ A,
-                                                            double *__restrict__ Lb, double *__restrict__ Db,
-                                                            long long nb, int nsteps, double dmin, double inv_delta,
-                                                            int *__restrict__ piv) {
-  __shared__ double lp[MICRO_PANEL][MICRO_ROWS];
-  __shared__ int s_done;
-  if (threadIdx.x == 0) s_done = st->done;
-  __syncthreads();
-  if (s_done) return;
+// (j, e, val) carry the loop state; lp[step][i] receives the panel's columns.
+__device__ void micro_panel_steps(double (&lp)[MICRO_PANEL][MICRO_ROWS], const double *__restrict__ A,
+                                  double *__restrict__ Lb, double *__restrict__ Db, long long nb, int nsteps,
+                                  double dmin, double inv_delta, int *__restrict__ piv, int &j, int &e, double &val,
+                                  int &done) {
   const int i = threadIdx.x;
   const bool row = i < nb;
-  int j = st->j, e = st->e;
-  double val = st->val;
-  const int p0 = j;
-  int done = 0;
+  done = 0;
   for (int step = 0; step < MICRO_PANEL; ++step) {
     const double sq = sqrt(val);
     double bk = inv_delta > 0.0 ? qabs(-1.0, inv_delta) : -1.0, bv = -1.0;
@@ -270,12 +264,90 @@ __global__ void __launch_bounds__(MICRO_ROWS) k_micro_panel(MicroState *st, cons
     done = (bi < 0 || bv <= dmin || j >= nb || j >= nsteps) ? 1 : 0;
     if (done) break;
   }
+}
+
+__global__ void __launch_bounds__(MICRO_ROWS) k_micro_panel(MicroState *st, const double *__restrict__ A,
+                                                            double *__restrict__ Lb, double *__restrict__ Db,
+                                                            long long nb, int nsteps, double dmin, double inv_delta,
+                                                            int *__restrict__ piv) {
+  __shared__ double lp[MICRO_PANEL][MICRO_ROWS];
+  __shared__ int s_done;
+  if (threadIdx.x == 0) s_done = st->done;
+  __syncthreads();
+  if (s_done) return;
+  int j = st->j, e = st->e, done = 0;
+  double val = st->val;
+  const int p0 = j;
+  micro_panel_steps(lp, A, Lb, Db, nb, nsteps, dmin, inv_delta, piv, j, e, val, done);
   if (threadIdx.x == 0) {
     st->p0 = p0;
     st->j = j;
     st->e = e;
     st->val = val;
     st->done = done;
+  }
+}
+
+// A(i, c) -= sum_{q < np} lp[q][i] lp[q][c] over the nb x nb block, the panel
+// columns read from LDS: 16 x 16 tiles of FP64 16x16x4 MFMA, one wavefront
+// per tile (MICRO_ROWS / 64 of them in turn).  The update is symmetric, so
+// the instruction's first operand takes the tile's columns c and its second
+// the rows i; lane l then holds rows i0 + (l & 15) of columns
+// c0 + 4 qq + (l >> 4), sixteen consecutive doubles of A per column.
+__device__ void micro_update_lds(const double (&lp)[MICRO_PANEL][MICRO_ROWS], double *__restrict__ A, long long nb,
+                                 int np) {
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63, n = l & 15, g = l >> 4;
+  const int ntr = (int)((nb + 15) / 16);
+  const int kst = (np + 3) / 4;
+  for (int t = w; t < ntr * ntr; t += MICRO_ROWS / 64) {
+    const int ti = t % ntr, tc = t / ntr;
+    const int i = 16 * ti + n, cm = 16 * tc + n;
+    d4 acc = {0.0, 0.0, 0.0, 0.0};
+    for (int ks = 0; ks < kst; ++ks) {
+      const int q = 4 * ks + g;
+      const double la = lp[q][cm < nb ? cm : 0], lb = lp[q][i < nb ? i : 0];
+      const double a = (q < np && cm < nb) ? la : 0.0;
+      const double b = (q < np && i < nb) ? lb : 0.0;
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int qq = 0; qq < 4; ++qq) {
+      const int c = 16 * tc + 4 * qq + g;
+      if (i < nb && c < nb) A[i + nb * c] -= acc[qq];
+    }
+  }
+}
+
+// All micro iterations of one macro iteration in one launch: panels of
+// MICRO_PANEL steps, each followed by its rank update of the batch block from
+// LDS, until the loop stops (at most ceil(nsteps / MICRO_PANEL) panels).
+// For batch blocks up to a size where one workgroup's update is cheaper than
+// two launches per panel (and a state read-back every few panels).
+__global__ void __launch_bounds__(MICRO_ROWS) k_micro_all(MicroState *st, double *__restrict__ A,
+                                                          double *__restrict__ Lb, double *__restrict__ Db,
+                                                          long long nb, int nsteps, double dmin, double inv_delta,
+                                                          int *__restrict__ piv) {
+  __shared__ double lp[MICRO_PANEL][MICRO_ROWS];
+  __shared__ int s_done;
+  if (threadIdx.x == 0) s_done = st->done;
+  __syncthreads();
+  if (s_done) return;
+  int j = st->j, e = st->e, done = 0, p0 = j;
+  double val = st->val;
+  const int npanel = (nsteps + MICRO_PANEL - 1) / MICRO_PANEL;
+  for (int pnl = 0; pnl < npanel; ++pnl) {
+    p0 = j;
+    micro_panel_steps(lp, A, Lb, Db, nb, nsteps, dmin, inv_delta, piv, j, e, val, done);
+    if (done) break;
+    micro_update_lds(lp, A, nb, j - p0);
+    __syncthreads();   // A updated, lp free for the next panel
+  }
+  if (threadIdx.x == 0) {
+    st->p0 = p0;
+    st->j = j;
+    st->e = e;
+    st->val = val;
+    st->done = 1;
   }
 }
 
@@ -465,14 +537,24 @@ static int pair_dim_of(const Context &C, int sa, int sb) {
   return sa != sb ? na * nb : na * (na + 1) / 2;
 }
 
+// a host array into a device buffer through the context's pinned arena
+template <class T>
+static void stage_upload(Context &C, DBuf<T> &b, const T *src, size_t count) {
+  b.alloc(count);
+  C.h2d(b.p, src, count * sizeof(T));
+}
+template <class T>
+static void stage_upload(Context &C, DBuf<T> &b, const std::vector<T> &v) {
+  stage_upload(C, b, v.data(), v.size());
+}
+
 static double device_max(Context &C, const double *d, long long n, DBuf<double> &tmp) {
   if (n <= 0) return -INFINITY;
   int nb = (int)std::min<long long>(1024, (n + 255) / 256);
   tmp.alloc(1024);
   hipLaunchKernelGGL(k_blockmax, dim3(nb), dim3(256), 0, C.stream, d, n, tmp.p);
   std::vector<double> h(nb);
-  GI_HIP(hipMemcpyAsync(h.data(), tmp.p, sizeof(double) * nb, hipMemcpyDeviceToHost, C.stream));
-  GI_HIP(hipStreamSynchronize(C.stream));
+  C.read(h.data(), tmp.p, sizeof(double) * nb);
   double m = -INFINITY;
   for (double v : h) m = v > m ? v : m;
   return m;
@@ -533,8 +615,7 @@ static void factor_and_invert(Context &C, DBuf<double> &V, long long n) {
   W.alloc((size_t)h * h);
   chol_inv_rec(C, V.p, X.p, n, 0, n, W.p, dinfo.p);
   int info = 0;
-  GI_HIP(hipMemcpyAsync(&info, dinfo.p, sizeof(info), hipMemcpyDeviceToHost, C.stream));
-  GI_HIP(hipStreamSynchronize(C.stream));
+  C.read(&info, dinfo.p, sizeof(info));
   if (info != 0) throw Error{40, "Cholesky factorization failed with exit code info=" + std::to_string(info)};
   std::swap(V.p, X.p);
   std::swap(V.n, X.n);
@@ -588,7 +669,7 @@ void chol_residual(Context &C, double *rmax, double *rmin) {
   GI_HIP(hipGetLastError());
   std::vector<double> r(ncol);
   GI_HIP(hipMemcpyAsync(r.data(), dout.p, sizeof(double) * ncol, hipMemcpyDeviceToHost, C.stream));
-  GI_HIP(hipStreamSynchronize(C.stream));
+  C.wait();
   double mx = -INFINITY, mn = INFINITY;
   for (long long c = 0; c < ncol; ++c)
     if (colfull[c] >= 0) {
@@ -666,7 +747,7 @@ void chol_build(Context &C, double tau, double kappa, long long max_block_dim, l
     R.out = d_vdiag.p; R.ldo = 0; R.kappa = kappa;
     eri_run(C, R);
     S.vdiag.resize(max_op);
-    GI_HIP(hipMemcpy(S.vdiag.data(), d_vdiag.p, sizeof(double) * max_op, hipMemcpyDeviceToHost));
+    C.read(S.vdiag.data(), d_vdiag.p, sizeof(double) * max_op);
     S.timings["vdiag"] = now() - t0;
   }
   const std::vector<double> &vdiag = S.vdiag;
@@ -784,7 +865,7 @@ void chol_build(Context &C, double tau, double kappa, long long max_block_dim, l
                  &dbm = cw.bm, &dMb = cw.Mb, &dLb = cw.Lb, &dDb = cw.Db, &dLP = cw.LP;
     DBuf<int> &dpiv = cw.piv;
     DBuf<MicroState> &dstate = cw.state;
-    dD.upload(Dh.data(), nop, C.stream);
+    stage_upload(C, dD, Dh.data(), (size_t)nop);
     long long ncand = nop;       // rows of L / length of D
     long long capL = 0;          // columns allocated in L
     dbm.alloc(2);
@@ -808,13 +889,12 @@ void chol_build(Context &C, double tau, double kappa, long long max_block_dim, l
         std::vector<long long> hl(n);
         std::vector<int> hd(n);
         for (long long k = 1; k <= n; ++k) { hl[k - 1] = loc_comp[cand[k]] - 1; hd[k - 1] = (int)spdim[cand[k]]; }
-        dloc.upload(hl, C.stream);
-        ddim.upload(hd, C.stream);
+        stage_upload(C, dloc, hl);
+        stage_upload(C, ddim, hd);
         dtmp.alloc(n);
         hipLaunchKernelGGL(k_segmax, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, C.stream, dD.p, dloc.p,
                            ddim.p, n, dtmp.p);
-        GI_HIP(hipMemcpyAsync(mv.data() + 1, dtmp.p, sizeof(double) * n, hipMemcpyDeviceToHost, C.stream));
-        GI_HIP(hipStreamSynchronize(C.stream));
+        C.read(mv.data() + 1, dtmp.p, sizeof(double) * n);
         for (long long k = 1; k <= n; ++k) {
           im[k] = cand[k];
           spmax[cand[k]] = mv[k];
@@ -847,7 +927,7 @@ void chol_build(Context &C, double tau, double kappa, long long max_block_dim, l
           for (long long i = 0; i < spdim[ab]; ++i) src.push_back(loc_comp[ab] - 1 + i);
           ncomp += spdim[ab];
         }
-        dsrc.upload(src, C.stream);
+        stage_upload(C, dsrc, src);
         dDn.alloc(ncomp);
         hipLaunchKernelGGL(k_gather_rows_of_cols, dim3((unsigned)((ncomp + 255) / 256), 1), dim3(256), 0, C.stream,
                            dDn.p, ncomp, dD.p, ncand, dsrc.p, ncomp, 1LL);
@@ -902,7 +982,7 @@ void chol_build(Context &C, double tau, double kappa, long long max_block_dim, l
           bd[colloc[k] - 1 + p] = loc_comp[bsp[k]] - 1 + p;
           bfull[colloc[k] - 1 + p] = loc_full[bsp[k]] + p;
         }
-      dbd.upload(bd, C.stream);
+      stage_upload(C, dbd, bd);
       t_batch += now() - tb0;
       // chol2_M: ERIs (candidates | batch)
       dM.alloc((size_t)std::max<long long>(ncand, 1) * std::max<long long>(max_nbatch, 1));
@@ -944,7 +1024,7 @@ void chol_build(Context &C, double tau, double kappa, long long max_block_dim, l
         else
           ROCBLAS_CHECK(rocblas_dgemm(C.blas, rocblas_operation_none, rocblas_operation_transpose, ncand,
                                       nops_batch, nchol, &alpha, dL.p, ncand, dLB.p, nops_batch, &beta, dM.p, ncand));
-        GI_HIP(hipStreamSynchronize(C.stream));
+        C.wait();
         t_rtr += now() - tr;
         rtr_flops += 2.0 * (double)ncand * (double)nops_batch * (double)nchol;
       }
@@ -972,13 +1052,20 @@ void chol_build(Context &C, double tau, double kappa, long long max_block_dim, l
         if (split) {
           std::vector<unsigned char> ex(nb);
           for (long long e = 0; e < nb; ++e) ex[e] = pivots[bfull[e]] ? 1 : 0;
-          dexcl.upload(ex, C.stream);
+          stage_upload(C, dexcl, ex);
         }
         hipLaunchKernelGGL(k_micro_init, dim3(1), dim3(1024), 0, C.stream, dD.p, dbd.p, nb, qd, dDb.p, dstate.p,
                            nsteps, split ? dMb.p : nullptr, split ? dexcl.p : nullptr);
-        // panels until the loop stops; the state is read back every 4 panels
+        // panels until the loop stops: in one launch for batch blocks up to
+        // micro_persist_max rows, else a panel and an update launch per
+        // MICRO_PANEL steps with the state read back every 4 panels
         const int npanel_max = (nsteps + MICRO_PANEL - 1) / MICRO_PANEL;
-        for (int pnl = 0; pnl < npanel_max;) {
+        if (nb <= C.micro_persist_max) {
+          hipLaunchKernelGGL(k_micro_all, dim3(1), dim3(MICRO_ROWS), 0, C.stream, dstate.p, dMb.p, dLb.p, dDb.p, nb,
+                             nsteps, dmin, qd, dpiv.p);
+          GI_HIP(hipGetLastError());
+        }
+        for (int pnl = nb <= C.micro_persist_max ? npanel_max : 0; pnl < npanel_max;) {
           for (int k = 0; k < 4 && pnl < npanel_max; ++k, ++pnl) {
             hipLaunchKernelGGL(k_micro_panel, dim3(1), dim3(MICRO_ROWS), 0, C.stream, dstate.p, dMb.p, dLb.p,
                                dDb.p, nb, nsteps, dmin, qd, dpiv.p);
@@ -986,17 +1073,15 @@ void chol_build(Context &C, double tau, double kappa, long long max_block_dim, l
                                C.stream, dstate.p, dMb.p, dLb.p, nb);
           }
           MicroState hs{};
-          GI_HIP(hipMemcpyAsync(&hs, dstate.p, sizeof(hs), hipMemcpyDeviceToHost, C.stream));
-          GI_HIP(hipStreamSynchronize(C.stream));
+          C.read(&hs, dstate.p, sizeof(hs));
           if (hs.done) break;
         }
         MicroState hs{};
-        GI_HIP(hipMemcpyAsync(&hs, dstate.p, sizeof(hs), hipMemcpyDeviceToHost, C.stream));
-        GI_HIP(hipStreamSynchronize(C.stream));
+        C.read(&hs, dstate.p, sizeof(hs));
         J = hs.j;
         std::vector<int> hpiv(std::max<long long>(J, 1));
         if (J > 0) {
-          GI_HIP(hipMemcpy(hpiv.data(), dpiv.p, sizeof(int) * J, hipMemcpyDeviceToHost));
+          C.read(hpiv.data(), dpiv.p, sizeof(int) * J);
           for (long long q = 0; q < J; ++q) {
             pivots[bfull[hpiv[q]]] = 1;
             pivot_order.push_back(bfull[hpiv[q]]);
@@ -1116,7 +1201,7 @@ void chol_build(Context &C, double tau, double kappa, long long max_block_dim, l
   std::vector<int> sel(nvec);
   for (long long i = 0; i < nvec; ++i) sel[i] = (int)(porb[i] - 1);
   DBuf<int> dsel;
-  dsel.upload(sel, C.stream);
+  stage_upload(C, dsel, sel);
   // chol2_Vpqrs
   S.inv_l.alloc((size_t)nvec * nvec);
   {
@@ -1168,7 +1253,7 @@ void chol_build(Context &C, double tau, double kappa, long long max_block_dim, l
       {
         std::vector<long long> gdim(ng);
         for (long long X = X0; X <= X1; ++X) gdim[X - X0] = sdim[X];
-        gpad.upload(gdim, C.stream);
+        stage_upload(C, gpad, gdim);
         hipLaunchKernelGGL(k_zero_pad_rows, dim3((unsigned)std::min<long long>(nvec, 1024), (unsigned)ng), dim3(64), 0,
                            C.stream, dW.p, ldw, msd, gpad.p, nvec);
         GI_HIP(hipGetLastError());
@@ -1218,14 +1303,14 @@ void chol_build(Context &C, double tau, double kappa, long long max_block_dim, l
                                     S.rkpq.p + (size_t)(X0 - 1) * nvec * msd + r0, nvec));
         r1 = r0;
       }
-      GI_HIP(hipStreamSynchronize(C.stream));
+      C.wait();
       t_gemm += now() - tg;
     }
     S.timings["vabrs"] = t_vab;
     S.timings["vectors_gemm"] = t_gemm;
     S.timings["vectors"] = now() - tf;
   }
-  GI_HIP(hipStreamSynchronize(C.stream));
+  C.wait();
   S.timings["total"] = now() - t_begin;
   S.nquartets = C.eri_quartets_acc;
   S.eri_flops = C.eri_flops_acc;

@@ -362,7 +362,10 @@ void eri_run(Context &C, const EriRequest &R) {
   long long nlaunch = 0;
   // device tables that outlive the request (pair headers, primitive records,
   // Hermite tables) grow first: the launch arguments below point into them
-  if (C.md_hdr_host.size() != nhdr0) C.md_hdr.upload(C.md_hdr_host, C.stream);
+  if (C.md_hdr_host.size() != nhdr0) {
+    C.md_hdr.alloc(C.md_hdr_host.size());
+    C.h2d(C.md_hdr.p, C.md_hdr_host.data(), C.md_hdr_host.size() * sizeof(PairHdr));
+  }
   if (!jobs.empty()) {
     C.md_prim.grow_keep((size_t)C.md_prim_used, C.stream);
     C.md_et.grow_keep((size_t)C.md_et_used, C.stream);
@@ -761,7 +764,7 @@ void eri_run(Context &C, const EriRequest &R) {
   C.eri_flops_acc += (long long)flops_total;
   C.eri_launches_acc += nlaunch;
   // the pair table and index buffers are reused by the next request
-  GI_HIP(hipStreamSynchronize(C.stream));
+  C.wait();
 }
 
 }  // namespace gi

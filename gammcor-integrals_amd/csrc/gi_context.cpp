@@ -1,6 +1,7 @@
 // Context lifetime, basis upload and error plumbing of libgigpu.
 #include "gi_context.h"
 #include <cstring>
+#include <algorithm>
 
 namespace gi {
 
@@ -63,6 +64,51 @@ Context::~Context() {
   if (stream) (void)hipStreamDestroy(stream);
   if (md_stage) (void)hipHostFree(md_stage);
   if (md_stage_ev) (void)hipEventDestroy(md_stage_ev);
+  if (wait_ev) (void)hipEventDestroy(wait_ev);
+  if (xfer) (void)hipHostFree(xfer);
+}
+
+void Context::wait() {
+  if (!blocking_waits) {
+    GI_HIP(hipStreamSynchronize(stream));
+  } else {
+    if (!wait_ev) GI_HIP(hipEventCreateWithFlags(&wait_ev, hipEventBlockingSync | hipEventDisableTiming));
+    GI_HIP(hipEventRecord(wait_ev, stream));
+    GI_HIP(hipEventSynchronize(wait_ev));
+  }
+  xfer_used = 0;   // every staged copy has completed
+}
+
+unsigned char *Context::xfer_slot(size_t bytes) {
+  const size_t need = (bytes + 255) & ~(size_t)255;
+  if (xfer_used + need > xfer_cap) {
+    if (xfer_used > 0) wait();   // copies out of the arena still in flight
+    if (need > xfer_cap) {
+      if (xfer) GI_HIP(hipHostFree(xfer));
+      xfer = nullptr;
+      const size_t cap = std::max(need, std::max<size_t>(1 << 20, 2 * xfer_cap));
+      GI_HIP(hipHostMalloc((void **)&xfer, cap, hipHostMallocDefault));
+      xfer_cap = cap;
+    }
+  }
+  unsigned char *p = xfer + xfer_used;
+  xfer_used += need;
+  return p;
+}
+
+void Context::h2d(void *dst, const void *src, size_t bytes) {
+  if (!bytes) return;
+  unsigned char *p = xfer_slot(bytes);
+  memcpy(p, src, bytes);
+  GI_HIP(hipMemcpyAsync(dst, p, bytes, hipMemcpyHostToDevice, stream));
+}
+
+void Context::read(void *dst, const void *src, size_t bytes) {
+  if (!bytes) return;
+  unsigned char *p = xfer_slot(bytes);
+  GI_HIP(hipMemcpyAsync(p, src, bytes, hipMemcpyDeviceToHost, stream));
+  wait();
+  memcpy(dst, p, bytes);
 }
 
 unsigned char *Context::stage_host(size_t n) {

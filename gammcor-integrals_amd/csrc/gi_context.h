@@ -159,9 +159,28 @@ struct Context {
   long long eri_md4_tiles = 2048;
   long long eri_lane_emax = LANE_EMAX;   // lane kernels only for classes with E, F <= this
   long long vec_blocks = 20;             // block rows of the triangular Inv_L Vabrs^T product
+  // micro iterations of a macro iteration in one launch (one workgroup) when
+  // the batch block has at most this many rows
+  long long micro_persist_max = 512;
   // class launches of one ERI request are spread over this many streams (the
   // context stream and eri_streams - 1 side streams forked from it and joined
   // back before the request returns); their destination blocks are disjoint
+  // host waits for the context stream: hipStreamSynchronize (a spin wait,
+  // the lowest latency) by default; with blocking_waits (the batched form,
+  // where many host threads wait at once) an event that sleeps until the GPU
+  // signals, so that waiting threads leave the host cores to the others
+  bool blocking_waits = false;
+  hipEvent_t wait_ev = nullptr;
+  void wait();
+  // small host <-> device transfers of the build through a pinned arena
+  // (grow-only): h2d stages the host data there and copies it on the context
+  // stream (the arena is reused once wait() has seen every copy complete);
+  // read copies device data to the host synchronously
+  unsigned char *xfer = nullptr;
+  size_t xfer_cap = 0, xfer_used = 0;
+  void h2d(void *dst, const void *src, size_t bytes);
+  void read(void *dst, const void *src, size_t bytes);
+  unsigned char *xfer_slot(size_t bytes);
   static constexpr int ERI_MAX_STREAMS = 4;
   long long eri_streams = ERI_MAX_STREAMS;
   hipStream_t eri_side[ERI_MAX_STREAMS - 1] = {};

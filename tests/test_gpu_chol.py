@@ -133,3 +133,28 @@ def test_range_separated_cholesky(gpu_ctx, oracle_mod):
     assert ref.nvecs < full.nvecs
     vecs = gammcor.chol_gammcor_Rkpq(b, gammcor.CHOL_ACCURACY_TIGHT, Omega=omega, ctx=gpu_ctx)
     assert vecs.Chol2Data.NVecs == ref.nvecs
+
+
+@pytest.mark.parametrize("n,basis", [(2, "large"), (4, "large")])
+def test_micro_iterations_one_launch(gpu_ctx, n, basis):
+    """The micro iterations of every macro iteration in one launch (panels and
+    their MFMA rank updates of the batch block from LDS) against a panel and
+    an update launch per 16 steps: the same pivots, hence the same Step-2
+    factorization and vectors."""
+    b = systems.water_system(n, 8.0, basis, seed=31)
+    gpu_ctx.set_basis(b)
+    res = {}
+    try:
+        for m in (0, 1024):
+            gpu_ctx.set_option("micro_persist_max", m)
+            gpu_ctx.chol_build(tau=1e-6)
+            res[m] = (np.array(gpu_ctx.chol_meta()["pivot_order"]), np.array(gpu_ctx.chol_rkpq()),
+                      gpu_ctx.chol_residual()[0])
+    finally:
+        gpu_ctx.set_option("micro_persist_max", 512)
+    assert np.array_equal(res[0][0], res[1024][0])
+    diff = float(np.abs(res[0][1] - res[1024][1]).max())
+    print("n=%d: %d vectors, max |R(one launch) - R(panels)| = %.2e, residual %.3e / %.3e"
+          % (n, len(res[0][0]), diff, res[0][2], res[1024][2]))
+    assert diff <= 1e-13 * float(np.abs(res[0][1]).max())
+    assert res[1024][2] == res[0][2]
