@@ -157,9 +157,9 @@ using Launch4Fn = void (*)(const MdArgs &, const ColTile *, long long, bool plai
 template <int E, int F>
 static void launch_md4(const MdArgs &a, const ColTile *ct, long long nct, bool plain, int grid, hipStream_t s) {
   if (plain)
-    hipLaunchKernelGGL((md4_kernel<E, F, true>), dim3((unsigned)grid), dim3(64), 0, s, a, ct, nct);
+    hipLaunchKernelGGL((md4_kernel<E, F, true>), dim3((unsigned)grid), dim3(64), 0, s, a, ct, nct, a.rrec);
   else
-    hipLaunchKernelGGL((md4_kernel<E, F, false>), dim3((unsigned)grid), dim3(64), 0, s, a, ct, nct);
+    hipLaunchKernelGGL((md4_kernel<E, F, false>), dim3((unsigned)grid), dim3(64), 0, s, a, ct, nct, a.rrec);
 }
 
 using Launch4sFn = void (*)(const MdArgs &, bool plain, int grid, hipStream_t);
@@ -167,9 +167,9 @@ using Launch4sFn = void (*)(const MdArgs &, bool plain, int grid, hipStream_t);
 template <int E>
 static void launch_md4s(const MdArgs &a, bool plain, int grid, hipStream_t s) {
   if (plain)
-    hipLaunchKernelGGL((md4s_kernel<E, true>), dim3((unsigned)grid), dim3(64), 0, s, a);
+    hipLaunchKernelGGL((md4s_kernel<E, true>), dim3((unsigned)grid), dim3(64), 0, s, a, a.rrec);
   else
-    hipLaunchKernelGGL((md4s_kernel<E, false>), dim3((unsigned)grid), dim3(64), 0, s, a);
+    hipLaunchKernelGGL((md4s_kernel<E, false>), dim3((unsigned)grid), dim3(64), 0, s, a, a.rrec);
 }
 static const Launch4sFn md4s_table[MD_EMAX + 1] = {&launch_md4s<0>, &launch_md4s<1>, &launch_md4s<2>, &launch_md4s<3>,
                                                    &launch_md4s<4>, &launch_md4s<5>, &launch_md4s<6>};

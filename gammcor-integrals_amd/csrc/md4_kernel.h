@@ -51,6 +51,16 @@ __device__ __forceinline__ int pick4(const int (&v)[4], int s) {
   return s == 0 ? v[0] : (s == 1 ? v[1] : (s == 2 ? v[2] : v[3]));
 }
 
+// a pair record at a uniform address, read into scalar registers
+__device__ __forceinline__ QRec load_rec_uniform(const QRec *__restrict__ p) {
+  QRec r;
+  const int *q = (const int *)p;
+  int *o = (int *)&r;
+#pragma unroll
+  for (int i = 0; i < (int)(sizeof(QRec) / 4); ++i) o[i] = __builtin_amdgcn_readfirstlane(q[i]);
+  return r;
+}
+
 // 64-bit pointer made provably uniform (scalar registers), so that loads
 // through it with a 32-bit lane offset use the scalar-base addressing form
 __device__ __forceinline__ const double *uniform_ptr(const double *p) {
@@ -112,7 +122,8 @@ __device__ __forceinline__ void md4_boys(const PrimRec &PB, const PrimRec &PK, d
 // PLAIN: every pair of the launch has one primitive pair (uncontracted), so the
 // kernel omits the primitive loops
 template <int E, int F, bool PLAIN>
-__global__ void __launch_bounds__(64) MD4_ATTR md4_kernel(MdArgs A, const ColTile *__restrict__ ctiles, long long nct) {
+__global__ void __launch_bounds__(64) MD4_ATTR md4_kernel(MdArgs A, const ColTile *__restrict__ ctiles, long long nct,
+                                                           const QRec *__restrict__ rrec) {
   using S = Md4<E, F>;
   constexpr int L = S::L, L1 = S::L1, NHE = S::NHE, NHF = S::NHF, NHL = S::NHL, NHG = S::NHG, KSF = S::KSF;
   constexpr int DS = S::DS, QM = S::QM, NL1 = S::NL1;
@@ -333,7 +344,7 @@ __global__ void __launch_bounds__(64) MD4_ATTR md4_kernel(MdArgs A, const ColTil
         md4_boys<L>(first_prim(RB), first_prim(CQ), A.kappa, A.boys, s_F[b][q], s_PQ[b][q]);
       }
       for (int b = 0; b < nb; ++b) {
-        const QRec RB = A.rrec[ir0 + b];
+        const QRec RB = load_rec_uniform(rrec + ir0 + b);
         fill_rows(RB);
         s2_open(RB, A.et + RB.et);
         load_rows(0);
@@ -349,7 +360,7 @@ __global__ void __launch_bounds__(64) MD4_ATTR md4_kernel(MdArgs A, const ColTil
       int maxnp = 1;
       for (int qq = 0; qq < nq; ++qq) maxnp = imax(maxnp, A.crec[pick4(T.jq, qq)].nprim);
       for (int b = 0; b < nb; ++b) {
-        const QRec RB = A.rrec[ir0 + b];
+        const QRec RB = load_rec_uniform(rrec + ir0 + b);
         fill_rows(RB);
         for (int bp = 0; bp < RB.nprim; ++bp) {
           const PrimRec PB = bp == 0 ? first_prim(RB) : A.prim[RB.prim0 + bp];
@@ -393,7 +404,7 @@ __global__ void __launch_bounds__(64) MD4_ATTR md4_kernel(MdArgs A, const ColTil
 constexpr int M4S_NK = 16;
 
 template <int E, bool PLAIN>
-__global__ void __launch_bounds__(64) MD4_ATTR md4s_kernel(MdArgs A) {
+__global__ void __launch_bounds__(64) MD4_ATTR md4s_kernel(MdArgs A, const QRec *__restrict__ rrec) {
   constexpr int L = E, NHE = nherm(E), NHG = (NHE + 3) / 4;
   constexpr int NL1 = L >= 1 ? nherm(L - 1) : 1;
   constexpr int SB = NHE | 1, SA = NL1 | 1;   // odd row strides (LDS banks)
@@ -548,7 +559,7 @@ __global__ void __launch_bounds__(64) MD4_ATTR md4s_kernel(MdArgs A) {
       }
       const double etv = cval ? A.et[CD.et + fc] : 0.0;
       for (int b = 0; b < nb; ++b) {
-        const QRec RB = A.rrec[ir0 + b];
+        const QRec RB = load_rec_uniform(rrec + ir0 + b);
         fill_rows(RB);
         s2_open(RB, A.et + RB.et);
         load_rows(0);
@@ -563,7 +574,7 @@ __global__ void __launch_bounds__(64) MD4_ATTR md4s_kernel(MdArgs A) {
       int maxnp = 1;
       for (int j = 0; j < nk; ++j) maxnp = imax(maxnp, A.crec[jc0 + j].nprim);
       for (int b = 0; b < nb; ++b) {
-        const QRec RB = A.rrec[ir0 + b];
+        const QRec RB = load_rec_uniform(rrec + ir0 + b);
         fill_rows(RB);
         for (int bp = 0; bp < RB.nprim; ++bp) {
           const PrimRec PB = bp == 0 ? first_prim(RB) : A.prim[RB.prim0 + bp];
