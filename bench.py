@@ -178,11 +178,14 @@ def precision_form(ctx, basis, Cmo, hip, reps: int = 3):
     return out
 
 
-def gradient_form(reps: int = 3):
+def gradient_form(reps: int = 3, large: bool = True):
     """The differentiation form on (H2O)8, O 6s4p2d / H 3s1p (320 AOs, 72
     coordinates), D = C_occ C_occ^T with nocc = 40: median seconds of the
     energy-only call and of the energy + nuclear gradient call, beside what
-    central differences over all coordinates would cost (2 energies each)."""
+    central differences over all coordinates would cost (2 energies each),
+    and what density-weighted Schwarz screening at 1e-12 changes; then one
+    energy and one energy + gradient call on the 1200-AO (H2O)16 system of
+    the headline, screened at 1e-12."""
     from gigpu import lib, systems
     basis = systems.config_gradient_cluster()
     D = systems.occupied_density(basis.nao, 40, seed=7)
@@ -198,17 +201,45 @@ def gradient_form(reps: int = 3):
             t0 = time.perf_counter()
             e2, g = ctx.coulomb_energy(D)
             tg.append(time.perf_counter() - t0)
+        # density-weighted Schwarz screening at 10^-12 on the same system:
+        # its effect on the result
+        ctx.set_option("grad_screen", 12)
+        es, gs = ctx.coulomb_energy(D)
+        ctx.set_option("grad_screen", 0)
     finally:
         ctx.close()
     nsh = basis.nshells
     npair = nsh * (nsh + 1) // 2
     t_e, t_g = float(np.median(te)), float(np.median(tg))
-    return {"workload": "E_J = 1/2 sum D_pq (pq|rs) D_rs and dE_J/dR of (H2O)8 (seed 5), O 6s4p2d / H 3s1p, "
-                        "D = C_occ C_occ^T (nocc=40, seed 7)",
-            "nao": basis.nao, "natoms": basis.natoms, "shell_pairs": npair,
-            "pair_products": npair * npair, "energy": e, "energy_s": t_e, "gradient_s": t_g,
-            "gradient_over_energy": t_g / t_e, "central_differences_s": 2 * 3 * basis.natoms * t_e,
-            "max_abs_gradient": float(np.abs(g).max())}
+    out = {"workload": "E_J = 1/2 sum D_pq (pq|rs) D_rs and dE_J/dR of (H2O)8 (seed 5), O 6s4p2d / H 3s1p, "
+                       "D = C_occ C_occ^T (nocc=40, seed 7)",
+           "nao": basis.nao, "natoms": basis.natoms, "shell_pairs": npair,
+           "pair_products": npair * npair, "energy": e, "energy_s": t_e, "gradient_s": t_g,
+           "gradient_over_energy": t_g / t_e, "central_differences_s": 2 * 3 * basis.natoms * t_e,
+           "max_abs_gradient": float(np.abs(g).max()),
+           "screened_1e-12": {"energy_diff": float(es - e2), "max_abs_gradient_diff": float(np.abs(gs - g).max())}}
+    if large:
+        # the (H2O)16 system of the headline (1200 AOs, O 8s5p3d1f / H 4s2p1d),
+        # screened at 10^-12 (unscreened: 22 s energy, 26 s energy + gradient)
+        lb = systems.water_system(16, 10.0, "large", seed=2)
+        LD = systems.occupied_density(lb.nao, 40, seed=7)
+        ctx = lib.Context(0)
+        try:
+            ctx.set_basis(lb)
+            ctx.set_option("grad_screen", 12)
+            t0 = time.perf_counter()
+            le = ctx.coulomb_energy(LD, grad=False)
+            lte = time.perf_counter() - t0
+            t0 = time.perf_counter()
+            le2, lg = ctx.coulomb_energy(LD)
+            ltg = time.perf_counter() - t0
+        finally:
+            ctx.close()
+        out["h2o16"] = {"workload": "the same quantities for (H2O)16 (seed 2), O 8s5p3d1f / H 4s2p1d, 1200 AOs, "
+                                    "D = C_occ C_occ^T (nocc=40, seed 7), Schwarz screening 1e-12",
+                        "nao": lb.nao, "natoms": lb.natoms, "energy": le2, "energy_s": lte, "gradient_s": ltg,
+                        "max_abs_gradient": float(np.abs(lg).max())}
+    return out
 
 
 def cpu_baseline(seed: int):

@@ -162,6 +162,9 @@ struct Context {
   // micro iterations of a macro iteration in one launch (one workgroup) when
   // the batch block has at most this many rows
   long long micro_persist_max = 512;
+  // Coulomb energy / gradient: shell pairs whose density-weighted Schwarz
+  // bound on every term is below 10^-grad_screen are left out (0: none)
+  long long grad_screen = 0;
   // class launches of one ERI request are spread over this many streams (the
   // context stream and eri_streams - 1 side streams forked from it and joined
   // back before the request returns); their destination blocks are disjoint

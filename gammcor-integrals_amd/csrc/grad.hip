@@ -39,6 +39,58 @@ __global__ void k_grad_recs(const PrimRec *__restrict__ prims, const int *__rest
   out[i] = QRec{pr.px, pr.py, pr.pz, pr.p, pr.et, GRAD_NCOL * i, nrow, 0, 0, GRAD_NCOL, 1, nprim[i], prim0[i], 0};
 }
 
+// shell pairs (s0 >= s1, s1-major order) kept by the screening of
+// coulomb_energy_grad: the Schwarz diagonals (ab|ab) of every pair from one
+// diagonal ERI request, the density maxima on the host
+static std::vector<char> screened_pairs(Context &C, const double *D, long long ldd, double kappa, double eps) {
+  const int nsh = C.nshells;
+  std::vector<PairDesc> rd, cd;
+  std::vector<int> rows;
+  long long loc = 0;
+  for (int s1 = 0; s1 < nsh; ++s1)
+    for (int s0 = s1; s0 < nsh; ++s0) {
+      const int fl = s0 == s1 ? PAIR_SAME : 0;
+      rd.push_back(PairDesc{s0, s1, fl, 0, 0, 0, loc});
+      cd.push_back(PairDesc{s0, s1, fl, 0, 0, 0, 0});
+      rows.push_back((int)rows.size());
+      const int n0 = C.shell_nang[s0], n1 = C.shell_nang[s1];
+      loc += s0 == s1 ? (long long)n0 * (n0 + 1) / 2 : (long long)n0 * n1;
+    }
+  DBuf<double> dq;
+  dq.alloc((size_t)loc);
+  EriRequest R;
+  R.rdesc = &rd;
+  R.cdesc = &cd;
+  R.rows = &rows;
+  R.cols = &rows;
+  R.mode = LAUNCH_DIAG;
+  R.out = dq.p;
+  R.ldo = 0;
+  R.kappa = kappa;
+  eri_run(C, R);
+  std::vector<double> q((size_t)loc);
+  C.read(q.data(), dq.p, sizeof(double) * (size_t)loc);
+  std::vector<double> w(rows.size());
+  double wsum = 0.0;
+  for (size_t i = 0; i < rows.size(); ++i) {
+    const int s0 = rd[i].s0, s1 = rd[i].s1;
+    const int n0 = C.shell_nang[s0], n1 = C.shell_nang[s1];
+    const long long o = rd[i].off, nr = s0 == s1 ? (long long)n0 * (n0 + 1) / 2 : (long long)n0 * n1;
+    double qm = 0.0, dm = 0.0;
+    for (long long r = 0; r < nr; ++r) qm = std::max(qm, q[(size_t)(o + r)]);
+    for (int b = 0; b < n1; ++b)
+      for (int a = 0; a < n0; ++a) {
+        const long long pa = C.shell_loc[s0] + a, pb = C.shell_loc[s1] + b;
+        dm = std::max(dm, std::max(std::fabs(D[pa + ldd * pb]), std::fabs(D[pb + ldd * pa])));
+      }
+    w[i] = std::sqrt(std::max(qm, 0.0)) * dm;
+    wsum += w[i];
+  }
+  std::vector<char> keep(rows.size());
+  for (size_t i = 0; i < rows.size(); ++i) keep[i] = w[i] * wsum >= eps ? 1 : 0;
+  return keep;
+}
+
 void coulomb_energy_grad(Context &C, const double *D, long long ldd, double kappa, double *energy,
                          double *grad) {
   if (!C.basis_set) throw Error{4, "basis not set"};
@@ -52,8 +104,16 @@ void coulomb_energy_grad(Context &C, const double *D, long long ldd, double kapp
   // R tables of order (E + 1) + F for the gradient: within the Boys table
   if (want_grad && 4 * C.lmax + 1 > BOYS_MAXM)
     throw Error{62, "the nuclear gradient needs 4 lmax + 1 <= " + std::to_string(BOYS_MAXM) + " (l <= 4)"};
+  // density-weighted Schwarz screening (grad_screen = v > 0, eps = 10^-v): with
+  // w_s = max_{ab in s} |D_ab| sqrt(max_{ab in s} (ab|ab)) for shell pair s,
+  // every term of E_J that involves s is at most w_s sum_t w_t, and the pair
+  // is left out when that is below eps (off by default: every pair kept)
+  std::vector<char> keep;
+  if (C.grad_screen > 0) keep = screened_pairs(C, D, ldd, kappa, std::pow(10.0, -(double)C.grad_screen));
+  long long ip = 0;
   for (int s1 = 0; s1 < nsh; ++s1)
-    for (int s0 = s1; s0 < nsh; ++s0) bins[C.shell_l[s0] + C.shell_l[s1]].push_back({s0, s1});
+    for (int s0 = s1; s0 < nsh; ++s0, ++ip)
+      if (keep.empty() || keep[ip]) bins[C.shell_l[s0] + C.shell_l[s1]].push_back({s0, s1});
   std::vector<long long> bin_off(nbin + 1, 0);
   for (int e = 0; e < nbin; ++e) bin_off[e + 1] = bin_off[e] + (long long)bins[e].size();
   const long long npair = bin_off[nbin];

@@ -114,6 +114,32 @@ def test_gradient_cluster_vs_differences(ctx):
     assert np.abs(g - g_fd).max() <= 1e-6, np.abs(g - g_fd).max()
 
 
+def test_screened_energy_and_gradient(ctx):
+    """Density-weighted Schwarz screening (grad_screen = v: shell pairs whose
+    bound on every term is below 10^-v are left out) against the unscreened
+    result on (H2O)8: measured differences 2.8e-14 (energy) and 6.8e-14
+    (gradient) at v = 14, 5.6e-12 and 9.0e-12 at v = 12; the bars allow ten
+    times that.  At v = 8 pairs are dropped (the energy moves) and the
+    result stays within 1e-6."""
+    basis = systems.config_gradient_cluster()
+    D = systems.occupied_density(basis.nao, 40, seed=7)
+    ctx.set_basis(basis)
+    e0, g0 = ctx.coulomb_energy(D)
+    out = {}
+    try:
+        for v in (14, 12, 8):
+            ctx.set_option("grad_screen", v)
+            out[v] = ctx.coulomb_energy(D)
+    finally:
+        ctx.set_option("grad_screen", 0)
+    for v, (de, dg) in ((14, (3e-13, 7e-13)), (12, (6e-11, 9e-11)), (8, (1e-6, 1e-5))):
+        e, g = out[v]
+        print("v=%d: dE %.2e max|dg| %.2e" % (v, e - e0, np.abs(g - g0).max()))
+        assert abs(e - e0) <= de, v
+        assert np.abs(g - g0).max() <= dg, v
+    assert out[8][0] != e0
+
+
 def test_autograd_function(ctx):
     import torch
     from gigpu.autograd import coulomb_energy
