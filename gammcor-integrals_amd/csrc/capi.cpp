@@ -519,4 +519,88 @@ int gi_ao_rows_to_auto2e_device_noalloc(gi_context *cp, int64_t ordering, const 
   })
 }
 
+int gi_thc_orbitals_device(gi_context *cp, int64_t npts, const double *xyz, int64_t normalize, double *Xgp,
+                           int64_t ldx) {
+  GI_TRY({
+    gi::Context &C = ctx_of(cp);
+    gi::thc_orbitals(C, xyz, npts, (int)(normalize != 0), Xgp, ldx);
+    C.wait();
+  })
+}
+
+int gi_thc_orbitals(gi_context *cp, int64_t npts, const double *xyz, int64_t normalize, double *Xgp,
+                    int64_t ldx) {
+  GI_TRY({
+    gi::Context &C = ctx_of(cp);
+    if (npts < 0 || ldx < npts) throw Error{81, "invalid number of grid points or leading dimension"};
+    if (npts == 0) return 0;
+    gi::DBuf<double> dxyz, dX;
+    dxyz.upload(xyz, (size_t)(3 * npts), C.stream);
+    dX.alloc((size_t)(ldx * C.nao));
+    gi::thc_orbitals(C, dxyz.p, npts, (int)(normalize != 0), dX.p, ldx);
+    GI_HIP(hipMemcpy2DAsync(Xgp, sizeof(double) * ldx, dX.p, sizeof(double) * ldx, sizeof(double) * npts, C.nao,
+                            hipMemcpyDeviceToHost, C.stream));
+    C.wait();
+  })
+}
+
+int gi_thc_z_device(gi_context *cp, int64_t npts, const double *Xgp, int64_t ldx, int64_t npts_reduced,
+                    double *Zgk, int64_t ldz, double *Zgk_reduced, int64_t ldzr) {
+  GI_TRY({
+    gi::Context &C = ctx_of(cp);
+    gi::thc_z(C, Xgp, ldx, npts, npts_reduced, Zgk, ldz, Zgk_reduced, ldzr);
+  })
+}
+
+int gi_thc_z(gi_context *cp, int64_t npts, const double *Xgp, int64_t ldx, int64_t npts_reduced, double *Zgk,
+             int64_t ldz, double *Zgk_reduced, int64_t ldzr) {
+  GI_TRY({
+    gi::Context &C = ctx_of(cp);
+    if (!C.chol.valid) throw Error{50, "Cholesky vectors are not available (run gi_chol_build first)"};
+    if (npts <= 0 || ldx < npts || ldz < npts) throw Error{81, "invalid number of grid points or leading dimension"};
+    const int64_t nvec = C.chol.nvecs;
+    const bool reduced = npts_reduced > 0 && npts_reduced < npts;
+    if (reduced && (!Zgk_reduced || ldzr < npts_reduced)) throw Error{81, "invalid ZgkReduced"};
+    gi::DBuf<double> dX, dZ, dZr;
+    dX.alloc((size_t)(npts * C.nao));
+    GI_HIP(hipMemcpy2DAsync(dX.p, sizeof(double) * npts, Xgp, sizeof(double) * ldx, sizeof(double) * npts, C.nao,
+                            hipMemcpyHostToDevice, C.stream));
+    dZ.alloc((size_t)(npts * nvec));
+    if (reduced) dZr.alloc((size_t)(npts_reduced * nvec));
+    gi::thc_z(C, dX.p, npts, npts, npts_reduced, dZ.p, npts, dZr.p, npts_reduced);
+    GI_HIP(hipMemcpy2DAsync(Zgk, sizeof(double) * ldz, dZ.p, sizeof(double) * npts, sizeof(double) * npts, nvec,
+                            hipMemcpyDeviceToHost, C.stream));
+    if (reduced)
+      GI_HIP(hipMemcpy2DAsync(Zgk_reduced, sizeof(double) * ldzr, dZr.p, sizeof(double) * npts_reduced,
+                              sizeof(double) * npts_reduced, nvec, hipMemcpyDeviceToHost, C.stream));
+    C.wait();
+  })
+}
+
+int gi_thc_rkab(gi_context *cp, int64_t npts, const double *Xgp, int64_t ldx, const double *Zgk, int64_t ldz,
+                int64_t nvec, const double *CA, int64_t lda, int64_t a0, int64_t a1, const double *CB,
+                int64_t ldb, int64_t b0, int64_t b1, int64_t ordering, double *Rkab) {
+  GI_TRY({
+    gi::Context &C = ctx_of(cp);
+    const int64_t NA = a1 - a0 + 1, NB = b1 - b0 + 1;
+    if (NA <= 0 || NB <= 0 || a0 < 1 || b0 < 1) throw Error{51, "Invalid dimension of Rkab"};
+    if (npts <= 0 || nvec <= 0 || ldx < npts || ldz < npts) throw Error{81, "invalid Xgp or Zgk"};
+    if (lda < C.nao || ldb < C.nao) throw Error{52, "Invalid number of atomic orbitals in CA/CB"};
+    gi::DBuf<double> dX, dZ, dA, dB, dR;
+    dX.alloc((size_t)(npts * C.nao));
+    GI_HIP(hipMemcpy2DAsync(dX.p, sizeof(double) * npts, Xgp, sizeof(double) * ldx, sizeof(double) * npts, C.nao,
+                            hipMemcpyHostToDevice, C.stream));
+    dZ.alloc((size_t)(npts * nvec));
+    GI_HIP(hipMemcpy2DAsync(dZ.p, sizeof(double) * npts, Zgk, sizeof(double) * ldz, sizeof(double) * npts, nvec,
+                            hipMemcpyHostToDevice, C.stream));
+    dA.upload(CA, (size_t)(lda * a1), C.stream);
+    dB.upload(CB, (size_t)(ldb * b1), C.stream);
+    const size_t nr = (size_t)(nvec * NA * NB);
+    dR.alloc(nr);
+    gi::thc_rkab(C, dX.p, npts, npts, dZ.p, npts, nvec, dA.p, lda, a0, a1, dB.p, ldb, b0, b1, (int)ordering, dR.p);
+    GI_HIP(hipMemcpyAsync(Rkab, dR.p, sizeof(double) * nr, hipMemcpyDeviceToHost, C.stream));
+    C.wait();
+  })
+}
+
 }  // extern "C"

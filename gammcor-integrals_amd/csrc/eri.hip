@@ -15,6 +15,7 @@
 #include "md4_kernel.h"
 #include "grad_kernels.h"
 #include "onee_kernels.h"
+#include "thc_kernels.h"
 
 namespace gi {
 

@@ -49,6 +49,7 @@ Context::~Context() {
   chol.mo_adj_col.release();
   chol.mo_adj_p.release();
   release_chol_work(*this);
+  release_thc_work(*this);
   for (int i = 0; i < 2; ++i) {
     spl_slices[i].release();
     spl_exp[i].release();

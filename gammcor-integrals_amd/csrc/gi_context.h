@@ -88,6 +88,7 @@ struct CholState {
 };
 
 struct CholWork;   // gi_chol_build work arrays (chol.hip)
+struct ThcWork;    // work arrays of the THC calls (thc.hip)
 
 struct Context {
   int device = 0;
@@ -133,6 +134,8 @@ struct Context {
   CholState chol;
   // device work arrays of gi_chol_build (chol.hip), grow-only across builds
   CholWork *cw = nullptr;
+  // device work arrays of the THC calls (thc.hip), grow-only
+  ThcWork *thcw = nullptr;
   // split-BF16 products (split.hip): BF16 slice planes and per-chunk row
   // scale exponents of the two operands; grow-only, kept across calls
   DBuf<unsigned short> spl_slices[2];
@@ -254,6 +257,17 @@ void launch_one_electron(const Context &C, int L, const int *ao_loc, const int *
                          const double *chg, int ncharge, double *S, double *T, double *V, long long ldh);
 void one_electron(Context &C, const double *atom_charges, long long npc, const double *pc_xyz, const double *pc_q,
                   double *S, double *T, double *V, long long ldh);
+
+// tensor hypercontraction (thc_kernels.h with eri.hip, thc.hip); device pointers
+void launch_thc_orbitals(const Context &C, int L, const int *ao_loc, const int *shl, long long ns,
+                         const double *xyz, long long npts, double *X, long long ldx);
+void thc_orbitals(Context &C, const double *dxyz, long long npts, int normalize, double *dX, long long ldx);
+void thc_z(Context &C, const double *dX, long long ldx, long long npts, long long nred, double *dZ, long long ldz,
+           double *dZr, long long ldzr);
+void thc_rkab(Context &C, const double *dX, long long ldx, long long npts, const double *dZ, long long ldz,
+              long long nvec, const double *dCA, long long lda, long long a0, long long a1, const double *dCB,
+              long long ldb, long long b0, long long b1, int ordering, double *dRkab);
+void release_thc_work(Context &C);
 
 // Coulomb energy of a density and its nuclear gradient (grad.hip)
 void coulomb_energy_grad(Context &C, const double *D, long long ldd, double kappa, double *energy,

@@ -197,6 +197,43 @@ module gigpu_c
       real(c_double), intent(in) :: CA(lda, *), CB(ldb, *)
       real(c_double), intent(out) :: Rkab(*)
     end function
+    ! tensor hypercontraction: orbitals on caller-given grid points, the
+    ! least-squares Z of the context's Cholesky vectors, MO-basis vectors
+    integer(c_int) function gi_thc_orbitals(ctx, npts, xyz, normalize, Xgp, ldx) bind(C, name="gi_thc_orbitals")
+      import :: c_int, c_ptr, c_int64_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int64_t), value :: npts, normalize, ldx
+      real(c_double), intent(in) :: xyz(3, *)
+      real(c_double), intent(out) :: Xgp(ldx, *)
+    end function
+    integer(c_int) function gi_thc_orbitals_device(ctx, npts, xyz, normalize, Xgp, ldx) &
+        bind(C, name="gi_thc_orbitals_device")
+      import :: c_int, c_ptr, c_int64_t
+      type(c_ptr), value :: ctx, xyz, Xgp
+      integer(c_int64_t), value :: npts, normalize, ldx
+    end function
+    integer(c_int) function gi_thc_z(ctx, npts, Xgp, ldx, npts_reduced, Zgk, ldz, Zgk_reduced, ldzr) &
+        bind(C, name="gi_thc_z")
+      import :: c_int, c_ptr, c_int64_t, c_double
+      type(c_ptr), value :: ctx, Zgk_reduced
+      integer(c_int64_t), value :: npts, ldx, npts_reduced, ldz, ldzr
+      real(c_double), intent(in) :: Xgp(ldx, *)
+      real(c_double), intent(out) :: Zgk(ldz, *)
+    end function
+    integer(c_int) function gi_thc_z_device(ctx, npts, Xgp, ldx, npts_reduced, Zgk, ldz, Zgk_reduced, ldzr) &
+        bind(C, name="gi_thc_z_device")
+      import :: c_int, c_ptr, c_int64_t
+      type(c_ptr), value :: ctx, Xgp, Zgk, Zgk_reduced
+      integer(c_int64_t), value :: npts, ldx, npts_reduced, ldz, ldzr
+    end function
+    integer(c_int) function gi_thc_rkab(ctx, npts, Xgp, ldx, Zgk, ldz, nvec, CA, lda, a0, a1, CB, ldb, b0, b1, &
+        ordering, Rkab) bind(C, name="gi_thc_rkab")
+      import :: c_int, c_ptr, c_int64_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int64_t), value :: npts, ldx, ldz, nvec, lda, a0, a1, ldb, b0, b1, ordering
+      real(c_double), intent(in) :: Xgp(ldx, *), Zgk(ldz, *), CA(lda, *), CB(ldb, *)
+      real(c_double), intent(out) :: Rkab(*)
+    end function
   end interface
 
 contains

@@ -24,11 +24,11 @@ ARCH = "gfx950"
 COMMON = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH, "-Wno-pass-failed",
           "-Wno-unused-result", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
           "-I" + CSRC, "-I" + os.path.join(os.path.dirname(PKG), "include")]
-LIBS = ["-L/opt/rocm/lib", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"]
+LIBS = ["-L/opt/rocm/lib", "-lrocsolver", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"]
 
-SOURCES = ["eri.hip", "chol.hip", "mo.hip", "grad.hip", "split.hip", "onee.hip", "aomap.hip", "gi_tables.cpp",
+SOURCES = ["eri.hip", "chol.hip", "mo.hip", "grad.hip", "split.hip", "onee.hip", "aomap.hip", "thc.hip", "gi_tables.cpp",
            "gi_context.cpp", "sort.cpp", "capi.cpp"]
-HEADERS = ["gi_common.h", "gi_context.h", "md_kernel.h", "md4_kernel.h", "md_generic.h", "md_classes.inc", "grad_kernels.h", "onee_kernels.h"]
+HEADERS = ["gi_common.h", "gi_context.h", "md_kernel.h", "md4_kernel.h", "md_generic.h", "md_classes.inc", "grad_kernels.h", "onee_kernels.h", "thc_kernels.h"]
 
 
 def _compile(src: str, incremental: bool) -> str:

@@ -19,7 +19,8 @@ EXPORTS = ["gi_create", "gi_destroy", "gi_last_error", "gi_set_basis", "gi_eri_q
            "gi_mo_transform", "gi_mo_transform_device", "gi_eri_class_stats", "gi_eri_pair_block",
            "gi_set_option", "gi_chol_build_batch", "gi_coulomb_energy_grad",
            "gi_chol_residual", "gi_one_electron", "gi_h0", "gi_ao_map", "gi_ao_rows_to_auto2e_device",
-           "gi_ao_rows_to_auto2e_device_noalloc"]
+           "gi_ao_rows_to_auto2e_device_noalloc", "gi_thc_orbitals", "gi_thc_orbitals_device", "gi_thc_z",
+           "gi_thc_z_device", "gi_thc_rkab"]
 
 CHOL_DECODE_FIXED = 1
 CHOL_SKIP_VECTORS = 2
@@ -79,6 +80,12 @@ def load():
     L.gi_ao_map.argtypes = [C.c_void_p, i64, iptr, dptr]
     L.gi_ao_rows_to_auto2e_device.argtypes = [C.c_void_p, i64, C.c_void_p, i64, i64, C.c_void_p, i64]
     L.gi_ao_rows_to_auto2e_device_noalloc.argtypes = [C.c_void_p, i64, C.c_void_p, i64, i64, C.c_void_p, i64]
+    L.gi_thc_orbitals.argtypes = [C.c_void_p, i64, dptr, i64, dptr, i64]
+    L.gi_thc_orbitals_device.argtypes = [C.c_void_p, i64, C.c_void_p, i64, C.c_void_p, i64]
+    L.gi_thc_z.argtypes = [C.c_void_p, i64, dptr, i64, i64, dptr, i64, C.c_void_p, i64]
+    L.gi_thc_z_device.argtypes = [C.c_void_p, i64, C.c_void_p, i64, i64, C.c_void_p, i64, C.c_void_p, i64]
+    L.gi_thc_rkab.argtypes = [C.c_void_p, i64, dptr, i64, dptr, i64, i64, dptr, i64, i64, i64, dptr, i64, i64, i64,
+                              i64, dptr]
     for name in EXPORTS:
         getattr(L, name).restype = C.c_int if name != "gi_last_error" else C.c_char_p
     _lib = L
@@ -319,6 +326,47 @@ class Context:
         allocations (map and signs staged in the library's static array)."""
         _check(self.L.gi_ao_rows_to_auto2e_device_noalloc(self.h, ordering, C.c_void_p(X_ptr), ld_ext, ncols,
                                                           C.c_void_p(Y_ptr), ld_ao))
+
+    def thc_orbitals(self, xyz, normalize=True):
+        """gi_thc_orbitals: Xgp (npts, nao) of the points xyz (npts, 3), bohr."""
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+        n = xyz.shape[0]
+        X = np.zeros((n, self.basis.nao), order="F")
+        _check(self.L.gi_thc_orbitals(self.h, n, _p(xyz), int(bool(normalize)), _p(X), max(n, 1)))
+        return X
+
+    def thc_orbitals_device(self, npts, xyz_ptr, X_ptr, ldx, normalize=True):
+        _check(self.L.gi_thc_orbitals_device(self.h, npts, C.c_void_p(xyz_ptr), int(bool(normalize)),
+                                             C.c_void_p(X_ptr), ldx))
+
+    def thc_z(self, Xgp, npts_reduced=0):
+        """gi_thc_z: (Zgk, Zgk_reduced or None) for the context's Cholesky vectors."""
+        X = np.asfortranarray(Xgp, dtype=np.float64)
+        n = X.shape[0]
+        nv = self.chol_dims()["nvecs"]
+        Z = np.zeros((n, nv), order="F")
+        red = 0 < npts_reduced < n
+        Zr = np.zeros((npts_reduced, nv), order="F") if red else None
+        _check(self.L.gi_thc_z(self.h, n, _p(X), n, npts_reduced, _p(Z), n,
+                               Zr.ctypes.data_as(C.c_void_p) if red else None, npts_reduced if red else 1))
+        return Z, Zr
+
+    def thc_z_device(self, npts, X_ptr, ldx, Z_ptr, ldz, npts_reduced=0, Zr_ptr=0, ldzr=1):
+        _check(self.L.gi_thc_z_device(self.h, npts, C.c_void_p(X_ptr), ldx, npts_reduced, C.c_void_p(Z_ptr), ldz,
+                                      C.c_void_p(Zr_ptr) if Zr_ptr else None, ldzr))
+
+    def thc_rkab(self, Xgp, Zgk, CA, a0, a1, CB, b0, b1, ordering=0):
+        """gi_thc_rkab: Rkab (nvec, NA*NB), a fastest."""
+        X = np.asfortranarray(Xgp, dtype=np.float64)
+        Z = np.asfortranarray(Zgk, dtype=np.float64)
+        CA = np.asfortranarray(CA, dtype=np.float64)
+        CB = np.asfortranarray(CB, dtype=np.float64)
+        n, nv = Z.shape
+        NA, NB = a1 - a0 + 1, b1 - b0 + 1
+        out = np.zeros(nv * NA * NB)
+        _check(self.L.gi_thc_rkab(self.h, n, _p(X), X.shape[0], _p(Z), n, nv, _p(CA), CA.shape[0], a0, a1, _p(CB),
+                                  CB.shape[0], b0, b1, ordering, _p(out)))
+        return out.reshape(NA * NB, nv).T
 
 
 def accuracy_tau(accuracy: int) -> float:

@@ -136,6 +136,25 @@ def config_gradient_cluster(seed: int = 5) -> AOBasis:
     return water_system(8, 6.0 * 2.0 ** (1.0 / 3.0), "small", seed)
 
 
+def atom_centred_points(basis: AOBasis, radii=(0.35, 0.9, 1.8, 3.0), ndir: int = 8,
+                        seed: int = 0) -> np.ndarray:
+    """Synthetic THC grid points (npts, 3), bohr: around every atom, ndir
+    directions of a Fibonacci sphere (randomly rotated per atom and radius)
+    at each radius.  Stands in for the caller's pruned molecular grid."""
+    rng = np.random.default_rng(seed)
+    k = np.arange(ndir) + 0.5
+    z = 1.0 - 2.0 * k / ndir
+    phi = math.pi * (1.0 + 5.0 ** 0.5) * k
+    rxy = np.sqrt(1.0 - z * z)
+    dirs = np.stack([rxy * np.cos(phi), rxy * np.sin(phi), z], axis=1)
+    pts = []
+    for A in basis.atom_coords:
+        for r in radii:
+            q, _ = np.linalg.qr(rng.standard_normal((3, 3)))
+            pts.append(A + r * dirs @ q.T)
+    return np.concatenate(pts, axis=0)
+
+
 def occupied_density(nao: int, nocc: int, seed: int) -> np.ndarray:
     """D = C_occ C_occ^T from a random orthonormal C (QR of an N(0, 1)
     nao x nao matrix, seeded), C_occ its first nocc columns."""

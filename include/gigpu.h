@@ -251,6 +251,51 @@ int gi_ao_rows_to_auto2e_device(gi_context *ctx, int64_t ordering, const double 
 int gi_ao_rows_to_auto2e_device_noalloc(gi_context *ctx, int64_t ordering, const double *X_ext, int64_t ld_ext,
                                         int64_t ncols, double *X_ao, int64_t ld_ao);
 
+/* ---- tensor hypercontraction (THC) of the Cholesky-decomposed Coulomb
+ * matrix, the second consumer of the decomposition
+ * (thc_CoulombMatrix_QuadraticMemory, TensorHypercontraction.f90:232-272;
+ * thc_gammcor_Rkab, THC_Gammcor.f90:25-80).  The grid points are the
+ * caller's: the Becke parent grid and its pruning (thc_Grid,
+ * TensorHypercontraction.f90:300-390) are not part of this library. */
+
+/* Atomic orbitals on grid points, gridfunc_Orbitals (GridFunctions.f90:154-240)
+ * in the Auto2e AO order of the basis, followed when normalize != 0 by
+ * thc_normalize_Xgp (TensorHypercontraction.f90:600-637): every row divided
+ * by its 2-norm.  xyz (3, npts) bohr; Xgp (npts, nao) column-major, leading
+ * dimension ldx.  Fails when a normalized row is zero (the reference assumes
+ * the grid was screened so that it never is).  Host pointers. */
+int gi_thc_orbitals(gi_context *ctx, int64_t npts, const double *xyz, int64_t normalize, double *Xgp,
+                    int64_t ldx);
+/* Same with device pointers for xyz and Xgp. */
+int gi_thc_orbitals_device(gi_context *ctx, int64_t npts, const double *xyz, int64_t normalize, double *Xgp,
+                           int64_t ldx);
+
+/* Least-squares THC factor of the context's Cholesky vectors, thc_Z
+ * (TensorHypercontraction.f90:393-476) with the precomputed vectors
+ * (QuadraticMemory = .false.): Zgk (npts, NVecs) solves S Z = XR with
+ * XR = 2 Xgpq R^T over the stored orbital pairs, diagonal products halved
+ * (thc_XR, :479-577; thc_Xgpq, :640-769), and S(g,h) = ((Xgp Xgp^T)(g,h))^2
+ * (thc_S, :580-597).  When 0 < npts_reduced < npts, Zgk_reduced
+ * (npts_reduced, NVecs) solves the same system on the first npts_reduced
+ * points (:455-466); otherwise it is not written.  Needs gi_chol_build
+ * first.  Fails when S is exactly singular.  Host pointers. */
+int gi_thc_z(gi_context *ctx, int64_t npts, const double *Xgp, int64_t ldx, int64_t npts_reduced, double *Zgk,
+             int64_t ldz, double *Zgk_reduced, int64_t ldzr);
+/* Same with device pointers for Xgp, Zgk and Zgk_reduced. */
+int gi_thc_z_device(gi_context *ctx, int64_t npts, const double *Xgp, int64_t ldx, int64_t npts_reduced,
+                    double *Zgk, int64_t ldz, double *Zgk_reduced, int64_t ldzr);
+
+/* MO-basis THC vectors, the tail of thc_gammcor_Rkab (THC_Gammcor.f90:69-79):
+ * Xga = Xgp C_ao(:, a0:a1) (thc_gammcor_Xga, :175-191, C_ao = the rows of CA
+ * taken from the external AO order `ordering` to the Auto2e order with the
+ * Orca signs; GI_ORDERING_AUTO2E takes the rows as they are), likewise Xgb,
+ * and Rkab(k, a, b) = sum_g Zgk(g, k) Xga(g, a) Xgb(g, b), a fastest, nvec
+ * rows (thc_gammcor_Rkab_2, :82-106).  CA (nao, *) ld lda, CB ld ldb,
+ * 1-based columns.  Host pointers. */
+int gi_thc_rkab(gi_context *ctx, int64_t npts, const double *Xgp, int64_t ldx, const double *Zgk, int64_t ldz,
+                int64_t nvec, const double *CA, int64_t lda, int64_t a0, int64_t a1, const double *CB,
+                int64_t ldb, int64_t b0, int64_t b1, int64_t ordering, double *Rkab);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,162 @@
+"""THC on the GPU (gi_thc_orbitals, gi_thc_z, gi_thc_rkab) against the oracle
+(oracle/thc.py), and at the 1200-AO size through properties: orbitals
+point by point, the normal equations S Z = XR with XR rows from the
+independent MO-transform path, Rkab from the device X and Z."""
+import numpy as np
+import pytest
+
+import aomap
+import pyoracle
+import thc
+from gigpu import lib, systems
+from gigpu import thc as gthc
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    pyoracle.build()
+    c = lib.Context(0)
+    yield c
+    c.close()
+
+
+@pytest.mark.parametrize("basis,spher", [("large", True), ("large", False), ("gh", True)])
+def test_orbitals_vs_oracle(ctx, basis, spher):
+    b = systems.water_system(2, 6.0, basis, seed=11, spher_ao=spher, contracted=True)
+    ctx.set_basis(b)
+    pts = systems.atom_centred_points(b, ndir=10, seed=5)
+    print("orbitals", basis, spher, b.nao, len(pts), flush=True)
+    for norm in (False, True):
+        X = ctx.thc_orbitals(pts, normalize=norm)
+        Xo = thc.orbitals(b, pts, normalize=norm)
+        assert np.abs(X - Xo).max() <= 1e-13 * np.abs(Xo).max()
+
+
+def test_orbitals_zero_row_fails(ctx):
+    b = systems.water_system(1, 6.0, "small", seed=1)
+    ctx.set_basis(b)
+    far = np.array([[0.0, 0.0, 0.0], [1e4, 0.0, 0.0]])
+    with pytest.raises(lib.GigpuError, match="cannot be normalized"):
+        ctx.thc_orbitals(far, normalize=True)
+    X = ctx.thc_orbitals(far, normalize=False)
+    assert np.all(X[1] == 0.0)
+
+
+def _oracle_z(ctx, b, X, nred=0):
+    m = ctx.chol_meta()
+    cols = thc.storage_pairs(b, m["shell_pairs"], m["shell_pair_loc"], m["max_subset_dim"], m["nsubsets"])
+    return thc.z(X, cols, ctx.chol_rkpq(), nred)
+
+
+@pytest.mark.parametrize("ndir,tol", [(6, 1e-9), (12, 1e-5)])
+def test_z_vs_oracle(ctx, ndir, tol):
+    """Z from the device's own decomposition against the oracle's dsysv
+    solve of the same system: LU and Bunch-Kaufman agree to about
+    cond(S) * 1e-16 (cond(S) ~ 1e5 for 72 points, ~ 1e9 for 144)."""
+    b = systems.water_system(1, 6.0, "small", seed=1)
+    ctx.set_basis(b)
+    ctx.chol_build(tau=1e-6)
+    pts = systems.atom_centred_points(b, ndir=ndir, seed=3)
+    print("z", ndir, len(pts), flush=True)
+    X = ctx.thc_orbitals(pts)
+    Z, Zr = ctx.thc_z(X, npts_reduced=len(pts) // 2)
+    Zo, Zro = _oracle_z(ctx, b, X, len(pts) // 2)
+    assert np.abs(Z - Zo).max() <= tol * np.abs(Zo).max()
+    assert np.abs(Zr - Zro).max() <= tol * np.abs(Zro).max()
+    # npts_reduced >= npts: no reduced system
+    Z2, Zr2 = ctx.thc_z(X, npts_reduced=len(pts))
+    assert Zr2 is None and np.array_equal(Z2, Z)
+
+
+def test_z_contracted_with_range_separation(ctx):
+    b = systems.water_system(2, 6.0, "small", seed=4, contracted=True)
+    ctx.set_basis(b)
+    ctx.chol_build(tau=1e-7, kappa=1.0 / 0.4 ** 2)
+    pts = systems.atom_centred_points(b, ndir=5, seed=8)
+    X = ctx.thc_orbitals(pts)
+    Z, _ = ctx.thc_z(X)
+    Zo, _ = _oracle_z(ctx, b, X)
+    assert np.abs(Z - Zo).max() <= 1e-8 * np.abs(Zo).max()
+
+
+@pytest.mark.parametrize("ordering", [aomap.AUTO2E, aomap.ORCA, aomap.PYSCF])
+def test_rkab_vs_oracle(ctx, ordering):
+    b = systems.water_system(2, 6.0, "large", seed=12)
+    ctx.set_basis(b)
+    ctx.chol_build(tau=1e-5)
+    pts = systems.atom_centred_points(b, ndir=6, seed=9)
+    X = ctx.thc_orbitals(pts)
+    Z, _ = ctx.thc_z(X)
+    rng = np.random.default_rng(ordering)
+    CA = rng.standard_normal((b.nao, 12))
+    CB = rng.standard_normal((b.nao, 9))
+    got = ctx.thc_rkab(X, Z, CA, 3, 12, CB, 2, 9, ordering)
+    ref = thc.rkab(X, Z, thc.coefficients_to_auto2e(b, CA, ordering), 3, 12,
+                   thc.coefficients_to_auto2e(b, CB, ordering), 2, 9)
+    assert got.shape == (Z.shape[1], 10 * 8)
+    assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max()
+
+
+def test_python_front_end(ctx):
+    b = systems.water_system(1, 6.0, "small", seed=1)
+    pts = systems.atom_centred_points(b, ndir=6, seed=3)
+    g = gthc.thc_CoulombMatrix(b, pts, 1e-6, NGridReduced=30, ctx=ctx)
+    assert g.NGrid == len(pts) and g.NGridReduced == 30 and g.ZgkReduced.shape[0] == 30
+    assert np.allclose(g.Zgh, g.Zgk @ g.Zgk.T, rtol=0, atol=1e-12 * np.abs(g.Zgh).max())
+    CA = np.random.default_rng(2).standard_normal((b.nao, 6))
+    R = gthc.thc_gammcor_Rkab(CA, 1, 6, CA, 1, 6, b, pts, gthc.THC_ACCURACY_TIGHT, lib.ORDERING_ORCA, ctx=ctx)
+    assert R.shape[1] == 36
+    # symmetric in (a, b) for CA = CB
+    nv = R.shape[0]
+    R3 = R.reshape(nv, 6, 6)
+    assert np.abs(R3 - R3.transpose(0, 2, 1)).max() <= 1e-12 * np.abs(R3).max()
+    with pytest.raises(gthc.THCError):
+        gthc.thc_gammcor_XZ(b, pts, 9, ctx=ctx)
+
+
+def test_z_requires_decomposition():
+    b = systems.water_system(1, 6.0, "small", seed=1)
+    c = lib.Context(0)
+    try:
+        c.set_basis(b)
+        X = c.thc_orbitals(systems.atom_centred_points(b, ndir=4, seed=1))
+        with pytest.raises(lib.GigpuError, match="gi_chol_build"):
+            c.thc_z(X)
+    finally:
+        c.close()
+
+
+def test_full_size_normal_equations(ctx):
+    """(H2O)16, O 8s5p3d1f / H 4s2p1d, 1200 AOs, 48 atoms x 4 radii x 16
+    directions = 3072 points: orbitals against the oracle, S Z = XR with XR
+    rows from gi_mo_transform (C = rows of X: Rkab(k, g, g) = x_g^T R_k x_g),
+    Rkab against the product of the device X and Z."""
+    b = systems.config_large_cluster()
+    ctx.set_basis(b)
+    ctx.chol_build(tau=1e-5)
+    pts = systems.atom_centred_points(b, ndir=16, seed=21)
+    print("full size", b.nao, len(pts), ctx.chol_dims()["nvecs"], flush=True)
+    X = ctx.thc_orbitals(pts)
+    Xo = thc.orbitals(b, pts)
+    assert np.abs(X - Xo).max() <= 1e-13
+    Z, _ = ctx.thc_z(X)
+    sel = np.array([0, 5, 700, 1500, 2222, 3071])
+    Cg = np.asfortranarray(X[sel].T)
+    Rk = ctx.mo_transform(Cg, 1, len(sel), Cg, 1, len(sel))
+    nv = Rk.shape[0]
+    XR = np.stack([Rk.reshape(nv, len(sel), len(sel))[:, i, i] for i in range(len(sel))])
+    Y = X[sel] @ X.T
+    SZ = (Y * Y) @ Z
+    # a backward-stable solve leaves a residual of a few eps |S| |Z| (measured
+    # 2.4e-13 against max|XR| = 1.4, max|Z| = 48 for this grid)
+    err, zmax = np.abs(SZ - XR).max(), np.abs(Z).max()
+    print("residual %.3e  max|XR| %.3e  max|Z| %.3e" % (err, np.abs(XR).max(), zmax), flush=True)
+    assert err <= 1e-10 * np.abs(XR).max()
+    rng = np.random.default_rng(4)
+    CA = rng.standard_normal((b.nao, 8))
+    got = ctx.thc_rkab(X, Z, CA, 1, 8, CA, 4, 7, 0)
+    Xa = X @ CA
+    ref = Z.T @ (Xa[:, 3:7, None] * Xa[:, None, :]).reshape(len(pts), -1)
+    assert np.abs(got - ref).max() <= 1e-11 * np.abs(ref).max()
