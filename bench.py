@@ -242,6 +242,120 @@ def gradient_form(reps: int = 3, large: bool = True):
     return out
 
 
+def thc_flops(npts, nvec, nao, npairs):
+    """Algorithmic FP64 operations of one least-squares Z: XR over the npairs
+    stored orbital pairs of the decomposition (the pairs that survive its
+    screening), S = (X X^T)^2, LU of S and the two triangular solves with
+    nvec right-hand sides."""
+    xr = 2.0 * npts * nvec * npairs
+    s = 2.0 * npts * npts * nao
+    solve = 2.0 / 3.0 * npts ** 3 + 2.0 * npts * npts * nvec
+    return xr, s, solve
+
+
+def thc_form(hip, reps: int = 3, cpu: bool = True):
+    """Tensor hypercontraction on the 1200-AO (H2O)16 system of the headline:
+    the decomposition at the THC default threshold (tau = 1e-5), 3072
+    atom-centred grid points (48 atoms x 4 radii x 16 directions), then the
+    orbitals on the grid, the least-squares Z and Rkab for nocc = 40 x
+    nvirt = 300, all on device pointers; median of reps calls each.  The CPU
+    leg times the oracle's Z (NumPy, threaded BLAS, LAPACK dsysv) on a bounded
+    sample, (H2O)4 with the same basis and grid rules (768 points), beside the
+    device on the same sample."""
+    from gigpu import lib, systems
+
+    def one(basis, npts_dir, reps_):
+        pts = np.ascontiguousarray(systems.atom_centred_points(basis, ndir=npts_dir, seed=21))
+        n, nao = len(pts), basis.nao
+        ctx = lib.Context(0)
+        bufs = []
+        try:
+            ctx.set_basis(basis)
+            ctx.chol_build(tau=1e-5)
+            dims = ctx.chol_dims()
+            nv = dims["nvecs"]
+            dxyz = hip.alloc(pts.nbytes)
+            hip.h2d(dxyz, pts)
+            dX, dZ = hip.alloc(8 * n * nao), hip.alloc(8 * n * nv)
+            bufs += [dxyz, dX, dZ]
+            t_orb, t_z = [], []
+            for _ in range(reps_ + 1):
+                t0 = time.perf_counter()
+                ctx.thc_orbitals_device(n, dxyz, dX, n)
+                t1 = time.perf_counter()
+                ctx.thc_z_device(n, dX, n, dZ, n)
+                t2 = time.perf_counter()
+                t_orb.append(t1 - t0)
+                t_z.append(t2 - t1)
+            res = {"nao": nao, "npts": n, "nvecs": nv, "norb_pairs": dims["norb_pairs"],
+                   "orbitals_s": float(np.median(t_orb[1:])),
+                   "z_s": float(np.median(t_z[1:]))}
+            return ctx, res, pts, bufs
+        except Exception:
+            ctx.close()
+            for b in bufs:
+                hip.free(b)
+            raise
+
+    basis = systems.water_system(16, 10.0, "large", seed=2)
+    ctx, res, pts, bufs = one(basis, 16, reps)
+    try:
+        n, nao, nv = res["npts"], res["nao"], res["nvecs"]
+        xr, sf, sv = thc_flops(n, nv, nao, res["norb_pairs"])
+        res["z_tflops"] = (xr + sf + sv) / res["z_s"] / 1e12
+        res["z_frac_fp64_peak"] = res["z_tflops"] / FP64_PEAK_TFLOPS
+        res["z_flops"] = {"xr": xr, "metric": sf, "solve": sv}
+        nocc, nvirt = 40, 300
+        rng = np.random.default_rng(2)
+        Cmo = np.asfortranarray(np.linalg.qr(rng.normal(size=(nao, nocc + nvirt)))[0])
+        dC, dR = hip.alloc(Cmo.nbytes), hip.alloc(8 * nv * nocc * nvirt)
+        bufs += [dC, dR]
+        hip.h2d(dC, Cmo)
+        dX, dZ = bufs[1], bufs[2]
+        tr = []
+        for _ in range(reps + 1):
+            t0 = time.perf_counter()
+            ctx.thc_rkab_device(n, dX, n, dZ, n, nv, dC, nao, 1, nocc, dC, nao, nocc + 1, nocc + nvirt, dR)
+            tr.append(time.perf_counter() - t0)
+        res["rkab_s"] = float(np.median(tr[1:]))
+        rk_flops = 2.0 * n * nao * (nocc + nvirt) + 2.0 * nv * n * nocc * nvirt
+        res["rkab_tflops"] = rk_flops / res["rkab_s"] / 1e12
+    finally:
+        ctx.close()
+        for b in bufs:
+            hip.free(b)
+    out = {"workload": "THC of (H2O)16 (seed 2), O 8s5p3d1f / H 4s2p1d, 1200 AOs, tau=1e-5, 3072 atom-centred "
+                       "points (seed 21), Rkab nocc=40 x nvirt=300", **res}
+    if cpu:
+        # the CPU leg: the oracle's NumPy restatement, timed as the baseline
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import thc as thc_oracle
+        small = systems.water_system(4, 6.0, "large", seed=2)
+        sctx, sres, spts, sbufs = one(small, 16, reps)
+        try:
+            m = sctx.chol_meta()
+            rk = sctx.chol_rkpq()
+            X = sctx.thc_orbitals(spts)
+        finally:
+            sctx.close()
+            for b in sbufs:
+                hip.free(b)
+        cols = thc_oracle.storage_pairs(small, m["shell_pairs"], m["shell_pair_loc"], m["max_subset_dim"],
+                                        m["nsubsets"])
+        t0 = time.perf_counter()
+        thc_oracle.z(X, cols, rk)
+        t_cpu = time.perf_counter() - t0
+        xr, sf, sv = thc_flops(sres["npts"], sres["nvecs"], sres["nao"], sres["norb_pairs"])
+        out["cpu_sample"] = {"workload": "the oracle's Z (NumPy + threaded BLAS + LAPACK dsysv) for (H2O)4 (seed 2), "
+                                         "same basis, %d points" % sres["npts"],
+                             "cores": os.cpu_count() if "OMP_NUM_THREADS" not in os.environ
+                             else int(os.environ["OMP_NUM_THREADS"]),
+                             "nao": sres["nao"], "nvecs": sres["nvecs"], "cpu_z_s": t_cpu,
+                             "gpu_z_s": sres["z_s"], "cpu_tflops": (xr + sf + sv) / t_cpu / 1e12,
+                             "gpu_over_cpu": t_cpu / sres["z_s"]}
+    return out
+
+
 def cpu_baseline(seed: int):
     """The CPU restatement (oracle, C/OpenMP) on a bounded sample -- the full
     build of (H2O)5 with the headline basis -- and, beside it, the GPU build
@@ -318,6 +432,7 @@ def main():
     ap.add_argument("--batch-workers", type=int, default=16)
     ap.add_argument("--no-autotune", action="store_true", help="skip autotuning the batched form")
     ap.add_argument("--no-gradient", action="store_true", help="skip the differentiation-form timing")
+    ap.add_argument("--no-thc", action="store_true", help="skip the tensor-hypercontraction timing")
     ap.add_argument("--no-precision", action="store_true", help="skip the split-BF16 precision-form comparison")
     ap.add_argument("--no-decode-ties", action="store_true",
                     help="skip the fixed-decode and raw-tie builds beside the headline build")
@@ -464,6 +579,8 @@ def main():
         out["batched"] = batched_form(args.batch_workers, not args.no_autotune)
     if not args.no_gradient:
         out["gradient"] = gradient_form()
+    if not args.no_thc:
+        out["thc"] = thc_form(hip, cpu=not args.no_cpu_baseline)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.seed)
     if args.eri_stats:

@@ -603,4 +603,13 @@ int gi_thc_rkab(gi_context *cp, int64_t npts, const double *Xgp, int64_t ldx, co
   })
 }
 
+int gi_thc_rkab_device(gi_context *cp, int64_t npts, const double *Xgp, int64_t ldx, const double *Zgk,
+                       int64_t ldz, int64_t nvec, const double *CA, int64_t lda, int64_t a0, int64_t a1,
+                       const double *CB, int64_t ldb, int64_t b0, int64_t b1, int64_t ordering, double *Rkab) {
+  GI_TRY({
+    gi::Context &C = ctx_of(cp);
+    gi::thc_rkab(C, Xgp, ldx, npts, Zgk, ldz, nvec, CA, lda, a0, a1, CB, ldb, b0, b1, (int)ordering, Rkab);
+  })
+}
+
 }  // extern "C"

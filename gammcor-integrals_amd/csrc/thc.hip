@@ -38,7 +38,7 @@ namespace gi {
 struct ThcWork {
   DBuf<int> shl, aoloc, flag, ipiv, info;
   DBuf<int> colpq;                    // (p, q) of every stored Rkpq column, -1 for padding
-  DBuf<double> W, S, Sr, CA, CB, Xa, Xb;
+  DBuf<double> W, S, Sr, CA, CB, Xa, Xb, part;
 };
 static ThcWork &thc_work(Context &C) {
   if (!C.thcw) C.thcw = new ThcWork();
@@ -50,24 +50,39 @@ void release_thc_work(Context &C) {
 }
 
 // ------------------------------------------------------------------ kernels
-// thc_normalize_Xgp: X(g, :) /= ||X(g, :)||_2 (scaled by the reciprocal like
-// the reference); one thread per point, so every column read is coalesced.
-// A point where every orbital is zero raises *flag and is left as it is.
-__global__ void __launch_bounds__(256) k_thc_rownorm(double *__restrict__ X, long long ldx, long long npts,
-                                                     long long nao, int *__restrict__ flag) {
+// thc_normalize_Xgp: X(g, :) /= ||X(g, :)||_2, scaled by the reciprocal like
+// the reference.  The AOs are split in nchunk column ranges so that a launch
+// has ceil(npts / 256) x nchunk workgroups: k_thc_sumsq writes the partial
+// sums of squares part(g, c), k_thc_scale adds them in a fixed order (the
+// result does not depend on the launch) and scales its range.  A point where
+// every orbital is zero raises *flag and is left as it is.
+__global__ void __launch_bounds__(256) k_thc_sumsq(const double *__restrict__ X, long long ldx, long long npts,
+                                                   long long nao, long long cw, double *__restrict__ part) {
   const long long g = blockIdx.x * 256LL + threadIdx.x;
   if (g >= npts) return;
+  const long long p0 = blockIdx.y * cw, p1 = std::min(nao, p0 + cw);
   double s = 0.0;
-  for (long long p = 0; p < nao; ++p) {
+  for (long long p = p0; p < p1; ++p) {
     const double v = X[g + ldx * p];
     s += v * v;
   }
+  part[g + npts * blockIdx.y] = s;
+}
+
+__global__ void __launch_bounds__(256) k_thc_scale(double *__restrict__ X, long long ldx, long long npts,
+                                                   long long nao, long long cw, int nchunk,
+                                                   const double *__restrict__ part, int *__restrict__ flag) {
+  const long long g = blockIdx.x * 256LL + threadIdx.x;
+  if (g >= npts) return;
+  double s = 0.0;
+  for (int c = 0; c < nchunk; ++c) s += part[g + npts * c];
   if (!(s > 0.0)) {
-    *flag = 1;
+    if (blockIdx.y == 0) *flag = 1;
     return;
   }
   const double r = 1.0 / sqrt(s);
-  for (long long p = 0; p < nao; ++p) X[g + ldx * p] *= r;
+  const long long p0 = blockIdx.y * cw, p1 = std::min(nao, p0 + cw);
+  for (long long p = p0; p < p1; ++p) X[g + ldx * p] *= r;
 }
 
 // Xgpq(g, c) for the stored columns c0 .. c0 + nc - 1 (thc_Xgpq,
@@ -132,10 +147,14 @@ void thc_orbitals(Context &C, const double *dxyz, long long npts, int normalize,
   for (int L = 0; L <= MAXL; ++L)
     launch_thc_orbitals(C, L, w.aoloc.p, w.shl.p + off[L], off[L + 1] - off[L], dxyz, npts, dX, ldx);
   if (normalize) {
+    const long long nao = C.nao, cw = std::max<long long>(16, (nao + 63) / 64);
+    const int nchunk = (int)((nao + cw - 1) / cw);
     w.flag.alloc(1);
+    w.part.alloc((size_t)(npts * nchunk));
     GI_HIP(hipMemsetAsync(w.flag.p, 0, sizeof(int), C.stream));
-    hipLaunchKernelGGL(k_thc_rownorm, dim3((unsigned)((npts + 255) / 256)), dim3(256), 0, C.stream, dX, ldx, npts,
-                       (long long)C.nao, w.flag.p);
+    const dim3 gn((unsigned)((npts + 255) / 256), (unsigned)nchunk);
+    hipLaunchKernelGGL(k_thc_sumsq, gn, dim3(256), 0, C.stream, dX, ldx, npts, nao, cw, w.part.p);
+    hipLaunchKernelGGL(k_thc_scale, gn, dim3(256), 0, C.stream, dX, ldx, npts, nao, cw, nchunk, w.part.p, w.flag.p);
     GI_HIP(hipGetLastError());
     int bad = 0;
     C.read(&bad, w.flag.p, sizeof(int));

@@ -234,6 +234,12 @@ module gigpu_c
       real(c_double), intent(in) :: Xgp(ldx, *), Zgk(ldz, *), CA(lda, *), CB(ldb, *)
       real(c_double), intent(out) :: Rkab(*)
     end function
+    integer(c_int) function gi_thc_rkab_device(ctx, npts, Xgp, ldx, Zgk, ldz, nvec, CA, lda, a0, a1, CB, ldb, &
+        b0, b1, ordering, Rkab) bind(C, name="gi_thc_rkab_device")
+      import :: c_int, c_ptr, c_int64_t
+      type(c_ptr), value :: ctx, Xgp, Zgk, CA, CB, Rkab
+      integer(c_int64_t), value :: npts, ldx, ldz, nvec, lda, a0, a1, ldb, b0, b1, ordering
+    end function
   end interface
 
 contains

@@ -20,7 +20,7 @@ EXPORTS = ["gi_create", "gi_destroy", "gi_last_error", "gi_set_basis", "gi_eri_q
            "gi_set_option", "gi_chol_build_batch", "gi_coulomb_energy_grad",
            "gi_chol_residual", "gi_one_electron", "gi_h0", "gi_ao_map", "gi_ao_rows_to_auto2e_device",
            "gi_ao_rows_to_auto2e_device_noalloc", "gi_thc_orbitals", "gi_thc_orbitals_device", "gi_thc_z",
-           "gi_thc_z_device", "gi_thc_rkab"]
+           "gi_thc_z_device", "gi_thc_rkab", "gi_thc_rkab_device"]
 
 CHOL_DECODE_FIXED = 1
 CHOL_SKIP_VECTORS = 2
@@ -86,6 +86,8 @@ def load():
     L.gi_thc_z_device.argtypes = [C.c_void_p, i64, C.c_void_p, i64, i64, C.c_void_p, i64, C.c_void_p, i64]
     L.gi_thc_rkab.argtypes = [C.c_void_p, i64, dptr, i64, dptr, i64, i64, dptr, i64, i64, i64, dptr, i64, i64, i64,
                               i64, dptr]
+    L.gi_thc_rkab_device.argtypes = [C.c_void_p, i64, C.c_void_p, i64, C.c_void_p, i64, i64, C.c_void_p, i64, i64,
+                                     i64, C.c_void_p, i64, i64, i64, i64, C.c_void_p]
     for name in EXPORTS:
         getattr(L, name).restype = C.c_int if name != "gi_last_error" else C.c_char_p
     _lib = L
@@ -367,6 +369,12 @@ class Context:
         _check(self.L.gi_thc_rkab(self.h, n, _p(X), X.shape[0], _p(Z), n, nv, _p(CA), CA.shape[0], a0, a1, _p(CB),
                                   CB.shape[0], b0, b1, ordering, _p(out)))
         return out.reshape(NA * NB, nv).T
+
+    def thc_rkab_device(self, npts, X_ptr, ldx, Z_ptr, ldz, nvec, CA_ptr, lda, a0, a1, CB_ptr, ldb, b0, b1,
+                        Rkab_ptr, ordering=0):
+        _check(self.L.gi_thc_rkab_device(self.h, npts, C.c_void_p(X_ptr), ldx, C.c_void_p(Z_ptr), ldz, nvec,
+                                         C.c_void_p(CA_ptr), lda, a0, a1, C.c_void_p(CB_ptr), ldb, b0, b1, ordering,
+                                         C.c_void_p(Rkab_ptr)))
 
 
 def accuracy_tau(accuracy: int) -> float:

@@ -295,6 +295,10 @@ int gi_thc_z_device(gi_context *ctx, int64_t npts, const double *Xgp, int64_t ld
 int gi_thc_rkab(gi_context *ctx, int64_t npts, const double *Xgp, int64_t ldx, const double *Zgk, int64_t ldz,
                 int64_t nvec, const double *CA, int64_t lda, int64_t a0, int64_t a1, const double *CB,
                 int64_t ldb, int64_t b0, int64_t b1, int64_t ordering, double *Rkab);
+/* Same with device pointers for Xgp, Zgk, CA, CB and Rkab. */
+int gi_thc_rkab_device(gi_context *ctx, int64_t npts, const double *Xgp, int64_t ldx, const double *Zgk,
+                       int64_t ldz, int64_t nvec, const double *CA, int64_t lda, int64_t a0, int64_t a1,
+                       const double *CB, int64_t ldb, int64_t b0, int64_t b1, int64_t ordering, double *Rkab);
 
 #ifdef __cplusplus
 }
