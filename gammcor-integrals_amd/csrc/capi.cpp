@@ -612,4 +612,23 @@ int gi_thc_rkab_device(gi_context *cp, int64_t npts, const double *Xgp, int64_t 
   })
 }
 
+int gi_thc_grid_pivots(gi_context *cp, int64_t npts, const double *xyz, double qr_thresh,
+                       double qr_thresh_reduced, int64_t block_dim, int64_t *npivots, int64_t *npivots_reduced,
+                       int64_t *pivots) {
+  GI_TRY({
+    gi::Context &C = ctx_of(cp);
+    if (npts <= 0) throw Error{81, "invalid number of grid points"};
+    if (!npivots || !npivots_reduced || !pivots) throw Error{7, "null output pointer"};
+    gi::DBuf<double> dxyz;
+    dxyz.upload(xyz, (size_t)(3 * npts), C.stream);
+    std::vector<long long> piv;
+    long long nred = 0;
+    gi::thc_grid_pivots(C, dxyz.p, npts, qr_thresh, qr_thresh_reduced, block_dim, piv, nred);
+    for (size_t i = 0; i < piv.size(); ++i) pivots[i] = (int64_t)piv[i];
+    *npivots = (int64_t)piv.size();
+    *npivots_reduced = (int64_t)nred;
+    C.wait();
+  })
+}
+
 }  // extern "C"

@@ -267,6 +267,8 @@ void thc_z(Context &C, const double *dX, long long ldx, long long npts, long lon
 void thc_rkab(Context &C, const double *dX, long long ldx, long long npts, const double *dZ, long long ldz,
               long long nvec, const double *dCA, long long lda, long long a0, long long a1, const double *dCB,
               long long ldb, long long b0, long long b1, int ordering, double *dRkab);
+void thc_grid_pivots(Context &C, const double *dxyz, long long npts, double qr_thresh, double qr_thresh_reduced,
+                     long long block_dim, std::vector<long long> &pivots, long long &nred);
 void release_thc_work(Context &C);
 
 // Coulomb energy of a density and its nuclear gradient (grad.hip)

@@ -6,8 +6,10 @@
 //                          (TensorHypercontraction.f90:393-597, real_linalg.f90:656-703)
 //   MO-basis vectors       thc_gammcor_Rkab tail: thc_gammcor_Xga, thc_gammcor_Rkab_2
 //                          (THC_Gammcor.f90:25-106, 175-191)
-// The grid points are the caller's (the Becke/Lebedev parent grid and its
-// pruning, thc_Grid, are not part of this library).
+//   grid pruning           thc_Chol_Pivots, thc_Chol_NextBlock, thc_Chol_Compress
+//                          (TensorHypercontraction.f90:772-1127)
+// The parent grid points are the caller's (the Becke/Lebedev grid of
+// thc_Grid is not part of this library).
 //
 // Z solves S Z = XR with
 //   XR(g, k) = 2 sum_{pq in storage} Xgpq(g, pq) R(k, pq),  Xgpq = X(g,p) X(g,q),
@@ -39,6 +41,12 @@ struct ThcWork {
   DBuf<int> shl, aoloc, flag, ipiv, info;
   DBuf<int> colpq;                    // (p, q) of every stored Rkpq column, -1 for padding
   DBuf<double> W, S, Sr, CA, CB, Xa, Xb, part;
+  // grid pruning: candidate coordinates and orbitals, residual diagonal,
+  // vectors over the candidates (two buffers: compression gathers into the
+  // other), staged pivot row, step state, pivots in order
+  DBuf<double> pxyz, pX, pD, pL[2], pxh, plh;
+  DBuf<long long> pcand, pmap, ppiv;
+  DBuf<unsigned char> pstate;
 };
 static ThcWork &thc_work(Context &C) {
   if (!C.thcw) C.thcw = new ThcWork();
@@ -124,6 +132,151 @@ __global__ void __launch_bounds__(256) k_thc_pair_products(const double *__restr
     const long long a = c % na, j = c / na;
     W[g + npts * c] = Xa[g + npts * a] * Xb[g + npts * j];
   }
+}
+
+// ---- grid pruning (pivoted Cholesky of S = (X X^T)^2 with on-the-fly columns)
+struct PivotState {
+  long long npiv;   // pivots taken so far (all blocks)
+  long long cur;    // vector index of the step's pivot
+  long long nred;   // NPivotsReduced
+  long long h;      // candidate index of the step's pivot
+  double dh;        // its residual diagonal
+  int done;         // no candidate above the threshold: every later step is skipped
+  int pad;
+};
+
+// One step's pivot: first candidate index of the largest D (maxloc), taken
+// when D > thresh (thc_Chol_NextBlock, TensorHypercontraction.f90:1012-1027);
+// its orbital row and its row of the earlier vectors are staged contiguously.
+// One workgroup of 1024 threads.
+__global__ void __launch_bounds__(1024) k_thc_pivot_select(const double *__restrict__ D, long long ncand,
+                                                           const double *__restrict__ X, long long nao,
+                                                           const double *__restrict__ L, double thresh,
+                                                           double thresh_red, const long long *__restrict__ cand,
+                                                           PivotState *st, long long *__restrict__ piv,
+                                                           double *__restrict__ xh, double *__restrict__ lh) {
+  __shared__ double s_v[1024];
+  __shared__ long long s_i[1024];
+  __shared__ int s_take;
+  __shared__ long long s_k;
+  const int t = threadIdx.x;
+  if (st->done) return;   // uniform: written only by thread 0 of an earlier launch
+  double bv = -1.0;
+  long long bi = -1;
+  for (long long g = t; g < ncand; g += 1024) {
+    const double v = D[g];
+    if (bi < 0 || v > bv) {   // strictly larger: the first index wins within a thread
+      bv = v;
+      bi = g;
+    }
+  }
+  s_v[t] = bv;
+  s_i[t] = bi;
+  __syncthreads();
+  for (int off = 512; off > 0; off >>= 1) {
+    if (t < off) {
+      const double v2 = s_v[t + off];
+      const long long i2 = s_i[t + off];
+      if (i2 >= 0 && (s_i[t] < 0 || v2 > s_v[t] || (v2 == s_v[t] && i2 < s_i[t]))) {
+        s_v[t] = v2;
+        s_i[t] = i2;
+      }
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    const double dh = s_v[0];
+    const int take = s_i[0] >= 0 && dh > thresh;
+    s_take = take;
+    if (take) {
+      const long long k = st->npiv;
+      s_k = k;
+      st->cur = k;
+      st->npiv = k + 1;
+      st->h = s_i[0];
+      st->dh = dh;
+      piv[k] = cand[s_i[0]];
+      if (dh > thresh_red) st->nred = k + 1;
+    } else {
+      st->done = 1;
+    }
+  }
+  __syncthreads();
+  if (!s_take) return;
+  const long long h = s_i[0], k = s_k;
+  for (long long p = t; p < nao; p += 1024) xh[p] = X[h + ncand * p];
+  for (long long j = t; j < k; j += 1024) lh[j] = L[h + ncand * j];
+}
+
+// The step's vector over every candidate g:
+//   L(g, k) = ((sum_p X(g,p) X(h,p))^2 - sum_{j<k} L(g,j) L(h,j)) / sqrt(D(h)),
+//   D(g) -= L(g, k)^2
+// 64 candidates per workgroup, the four wavefronts take every fourth AO and
+// every fourth earlier vector, partial sums added through LDS.
+__global__ void __launch_bounds__(256) k_thc_pivot_column(const double *__restrict__ X, long long ncand,
+                                                          long long nao, double *__restrict__ L,
+                                                          const double *__restrict__ xh,
+                                                          const double *__restrict__ lh, const PivotState *st,
+                                                          double *__restrict__ D) {
+  __shared__ double s1[4][64], s2[4][64];
+  if (st->done) return;
+  const long long k = st->cur;
+  const double dh = st->dh;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const long long g = blockIdx.x * 64LL + lane;
+  double a = 0.0, c = 0.0;
+  if (g < ncand) {
+    for (long long p = w; p < nao; p += 4) a += X[g + ncand * p] * xh[p];
+    for (long long j = w; j < k; j += 4) c += L[g + ncand * j] * lh[j];
+  }
+  s1[w][lane] = a;
+  s2[w][lane] = c;
+  __syncthreads();
+  if (w == 0 && g < ncand) {
+    const double sg = s1[0][lane] + s1[1][lane] + s1[2][lane] + s1[3][lane];
+    const double lg = s2[0][lane] + s2[1][lane] + s2[2][lane] + s2[3][lane];
+    const double v = (sg * sg - lg) / sqrt(dh);
+    L[g + ncand * k] = v;
+    D[g] -= v * v;
+  }
+}
+
+// compression (thc_Chol_Compress, :1067-1127): candidate rows map[i] of the
+// vectors, the residual diagonal, the candidate list and coordinates
+__global__ void __launch_bounds__(256) k_thc_compress(const long long *__restrict__ map, long long nnew,
+                                                      long long nold, long long nvec,
+                                                      const double *__restrict__ Lold, double *__restrict__ Lnew,
+                                                      const double *__restrict__ Dold, double *__restrict__ Dnew,
+                                                      const long long *__restrict__ cold,
+                                                      long long *__restrict__ cnew) {
+  const long long i = blockIdx.x * 256LL + threadIdx.x;
+  if (i >= nnew) return;
+  const long long src = map[i];
+  for (long long j = blockIdx.y; j < nvec; j += gridDim.y) Lnew[i + nnew * j] = Lold[src + nold * j];
+  if (blockIdx.y == 0) {
+    Dnew[i] = Dold[src];
+    cnew[i] = cold[src];
+  }
+}
+
+__global__ void k_thc_gather_xyz(const double *__restrict__ xyz, const long long *__restrict__ cand, long long n,
+                                 double *__restrict__ out) {
+  const long long i = blockIdx.x * 256LL + threadIdx.x;
+  if (i >= n) return;
+  const long long c = cand[i];
+  out[3 * i] = xyz[3 * c];
+  out[3 * i + 1] = xyz[3 * c + 1];
+  out[3 * i + 2] = xyz[3 * c + 2];
+}
+
+__global__ void k_thc_fill(double *__restrict__ p, long long n, double v) {
+  const long long i = blockIdx.x * 256LL + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__global__ void k_thc_iota(long long *__restrict__ p, long long n) {
+  const long long i = blockIdx.x * 256LL + threadIdx.x;
+  if (i < n) p[i] = i;
 }
 
 // ------------------------------------------------------------------ host
@@ -288,6 +441,95 @@ void thc_rkab(Context &C, const double *dX, long long ldx, long long npts, const
                            ldz, w.W.p, npts, &zero, dRkab + nvec * NA * b, nvec));
   }
   C.wait();
+}
+
+// thc_Chol_Pivots (TensorHypercontraction.f90:772-968) on the device: blocks of
+// at most block_dim pivots, each step one select and one column launch (a
+// step after the last pivot exits at once), one synchronization per block for
+// the compression.  dxyz: the parent points (3, npts), device.
+void thc_grid_pivots(Context &C, const double *dxyz, long long npts, double qr_thresh, double qr_thresh_reduced,
+                     long long block_dim, std::vector<long long> &pivots, long long &nred) {
+  if (!C.basis_set) throw Error{4, "basis not set"};
+  if (npts <= 0) throw Error{81, "invalid number of grid points"};
+  if (!(qr_thresh > 0.0)) throw Error{84, "invalid QR threshold"};
+  if (block_dim <= 0) throw Error{84, "invalid block dimension"};
+  ThcWork &w = thc_work(C);
+  const long long nao = C.nao;
+  const double thresh = qr_thresh * qr_thresh;
+  const double thresh_red = qr_thresh_reduced > 0.0 ? qr_thresh_reduced * qr_thresh_reduced : -1.0;
+  const long long max_iters = npts / block_dim + (npts % block_dim > 0);
+  long long ncand = npts, npiv = 0;
+  w.pcand.alloc((size_t)npts);
+  w.pmap.alloc((size_t)npts);
+  w.pD.alloc((size_t)npts);
+  w.ppiv.alloc((size_t)npts);
+  w.pxh.alloc((size_t)nao);
+  w.plh.alloc((size_t)std::max<long long>(npts, 1));
+  w.pstate.alloc(sizeof(PivotState));
+  w.pxyz.alloc((size_t)(3 * npts));
+  const unsigned g1 = (unsigned)((npts + 255) / 256);
+  hipLaunchKernelGGL(k_thc_iota, dim3(g1), dim3(256), 0, C.stream, w.pcand.p, npts);
+  hipLaunchKernelGGL(k_thc_fill, dim3(g1), dim3(256), 0, C.stream, w.pD.p, npts, 1.0);
+  PivotState st0{};
+  C.h2d(w.pstate.p, &st0, sizeof(st0));
+  PivotState *dst = (PivotState *)w.pstate.p;
+  int cur = 0;
+  bool converged = false;
+  std::vector<double> hD;
+  std::vector<long long> map;
+  for (long long it = 0; it < max_iters; ++it) {
+    // orbitals of the candidates, normalized (thc_Chol_Pivots :893-895)
+    hipLaunchKernelGGL(k_thc_gather_xyz, dim3((unsigned)((ncand + 255) / 256)), dim3(256), 0, C.stream, dxyz,
+                       w.pcand.p, ncand, w.pxyz.p);
+    w.pX.alloc((size_t)(ncand * nao));
+    thc_orbitals(C, w.pxyz.p, ncand, 1, w.pX.p, ncand);
+    const long long max_j = std::min(block_dim, npts - npiv);
+    w.pL[cur].grow_keep((size_t)(ncand * (npiv + max_j)), C.stream);
+    // the state's done flag starts clear for every block
+    st0.npiv = npiv;
+    C.h2d(w.pstate.p, &st0, sizeof(st0));
+    const unsigned gc = (unsigned)((ncand + 63) / 64);
+    for (long long k = 0; k < max_j; ++k) {
+      hipLaunchKernelGGL(k_thc_pivot_select, dim3(1), dim3(1024), 0, C.stream, w.pD.p, ncand, w.pX.p, nao,
+                         w.pL[cur].p, thresh, thresh_red, w.pcand.p, dst, w.ppiv.p, w.pxh.p, w.plh.p);
+      hipLaunchKernelGGL(k_thc_pivot_column, dim3(gc), dim3(256), 0, C.stream, w.pX.p, ncand, nao, w.pL[cur].p,
+                         w.pxh.p, w.plh.p, dst, w.pD.p);
+    }
+    GI_HIP(hipGetLastError());
+    C.read(&st0, w.pstate.p, sizeof(st0));
+    npiv = st0.npiv;
+    nred = st0.nred;
+    st0.done = 0;
+    // compression: keep the candidates with D > thresh, in order
+    hD.resize((size_t)ncand);
+    C.read(hD.data(), w.pD.p, sizeof(double) * ncand);
+    map.clear();
+    for (long long g = 0; g < ncand; ++g)
+      if (hD[g] > thresh) map.push_back(g);
+    if (map.empty()) {
+      converged = true;
+      break;
+    }
+    const long long nnew = (long long)map.size();
+    w.pmap.upload(map, C.stream);
+    const int nxt = 1 - cur;
+    w.pL[nxt].alloc((size_t)(nnew * std::max<long long>(npiv, 1)));
+    DBuf<double> Dn;
+    DBuf<long long> cn;
+    Dn.alloc((size_t)nnew);
+    cn.alloc((size_t)nnew);
+    hipLaunchKernelGGL(k_thc_compress, dim3((unsigned)((nnew + 255) / 256), ygrid(npiv)), dim3(256), 0, C.stream,
+                       w.pmap.p, nnew, ncand, npiv, w.pL[cur].p, w.pL[nxt].p, w.pD.p, Dn.p, w.pcand.p, cn.p);
+    GI_HIP(hipGetLastError());
+    GI_HIP(hipMemcpyAsync(w.pD.p, Dn.p, sizeof(double) * nnew, hipMemcpyDeviceToDevice, C.stream));
+    GI_HIP(hipMemcpyAsync(w.pcand.p, cn.p, sizeof(long long) * nnew, hipMemcpyDeviceToDevice, C.stream));
+    cur = nxt;
+    ncand = nnew;
+  }
+  if (!converged) throw Error{85, "Cholesky decomposistion failed to converge"};
+  if (!(qr_thresh_reduced > 0.0)) nred = npiv;
+  pivots.resize((size_t)npiv);
+  if (npiv > 0) C.read(pivots.data(), w.ppiv.p, sizeof(long long) * npiv);
 }
 
 }  // namespace gi

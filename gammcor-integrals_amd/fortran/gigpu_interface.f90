@@ -199,6 +199,15 @@ module gigpu_c
     end function
     ! tensor hypercontraction: orbitals on caller-given grid points, the
     ! least-squares Z of the context's Cholesky vectors, MO-basis vectors
+    integer(c_int) function gi_thc_grid_pivots(ctx, npts, xyz, qr_thresh, qr_thresh_reduced, block_dim, &
+        npivots, npivots_reduced, pivots) bind(C, name="gi_thc_grid_pivots")
+      import :: c_int, c_ptr, c_int64_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int64_t), value :: npts, block_dim
+      real(c_double), intent(in) :: xyz(3, *)
+      real(c_double), value :: qr_thresh, qr_thresh_reduced
+      integer(c_int64_t), intent(out) :: npivots, npivots_reduced, pivots(*)
+    end function
     integer(c_int) function gi_thc_orbitals(ctx, npts, xyz, normalize, Xgp, ldx) bind(C, name="gi_thc_orbitals")
       import :: c_int, c_ptr, c_int64_t, c_double
       type(c_ptr), value :: ctx

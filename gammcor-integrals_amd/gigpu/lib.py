@@ -20,7 +20,7 @@ EXPORTS = ["gi_create", "gi_destroy", "gi_last_error", "gi_set_basis", "gi_eri_q
            "gi_set_option", "gi_chol_build_batch", "gi_coulomb_energy_grad",
            "gi_chol_residual", "gi_one_electron", "gi_h0", "gi_ao_map", "gi_ao_rows_to_auto2e_device",
            "gi_ao_rows_to_auto2e_device_noalloc", "gi_thc_orbitals", "gi_thc_orbitals_device", "gi_thc_z",
-           "gi_thc_z_device", "gi_thc_rkab", "gi_thc_rkab_device"]
+           "gi_thc_z_device", "gi_thc_rkab", "gi_thc_rkab_device", "gi_thc_grid_pivots"]
 
 CHOL_DECODE_FIXED = 1
 CHOL_SKIP_VECTORS = 2
@@ -86,6 +86,7 @@ def load():
     L.gi_thc_z_device.argtypes = [C.c_void_p, i64, C.c_void_p, i64, i64, C.c_void_p, i64, C.c_void_p, i64]
     L.gi_thc_rkab.argtypes = [C.c_void_p, i64, dptr, i64, dptr, i64, i64, dptr, i64, i64, i64, dptr, i64, i64, i64,
                               i64, dptr]
+    L.gi_thc_grid_pivots.argtypes = [C.c_void_p, i64, dptr, C.c_double, C.c_double, i64, iptr, iptr, iptr]
     L.gi_thc_rkab_device.argtypes = [C.c_void_p, i64, C.c_void_p, i64, C.c_void_p, i64, i64, C.c_void_p, i64, i64,
                                      i64, C.c_void_p, i64, i64, i64, i64, C.c_void_p]
     for name in EXPORTS:
@@ -328,6 +329,16 @@ class Context:
         allocations (map and signs staged in the library's static array)."""
         _check(self.L.gi_ao_rows_to_auto2e_device_noalloc(self.h, ordering, C.c_void_p(X_ptr), ld_ext, ncols,
                                                           C.c_void_p(Y_ptr), ld_ao))
+
+    def thc_grid_pivots(self, xyz, qr_thresh, qr_thresh_reduced=-1.0, block_dim=500):
+        """gi_thc_grid_pivots: (pivots 0-based in pivot order, npivots_reduced)."""
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+        n = xyz.shape[0]
+        piv = np.zeros(max(n, 1), np.int64)
+        npv, nred = i64(), i64()
+        _check(self.L.gi_thc_grid_pivots(self.h, n, _p(xyz), qr_thresh, qr_thresh_reduced, block_dim,
+                                         C.byref(npv), C.byref(nred), _p(piv, i64)))
+        return piv[:npv.value].copy(), nred.value
 
     def thc_orbitals(self, xyz, normalize=True):
         """gi_thc_orbitals: Xgp (npts, nao) of the points xyz (npts, 3), bohr."""

@@ -258,6 +258,24 @@ int gi_ao_rows_to_auto2e_device_noalloc(gi_context *ctx, int64_t ordering, const
  * caller's: the Becke parent grid and its pruning (thc_Grid,
  * TensorHypercontraction.f90:300-390) are not part of this library. */
 
+/* Pruning of a parent grid by the pivots of S(g,h) = ((Xgp Xgp^T)(g,h))^2,
+ * thc_Chol_Pivots (TensorHypercontraction.f90:772-968) with thc_Chol_NextBlock
+ * (:971-1064) and thc_Chol_Compress (:1067-1127): a pivoted Cholesky of S with
+ * its columns built on the fly from the row-normalized orbitals, in blocks of
+ * at most block_dim pivots (THC_BlockDim, 500), the points whose residual
+ * diagonal falls to PivotThresh = qr_thresh^2 dropped after every block.
+ * xyz (3, npts) bohr, host.  On return pivots[0 .. *npivots - 1] hold the
+ * 0-based indices of the chosen points in pivot order (the reference returns
+ * their coordinates in that order) and *npivots_reduced the count of leading
+ * pivots whose diagonal exceeded qr_thresh_reduced^2 (NGridReduced; equal to
+ * *npivots when qr_thresh_reduced <= 0).  pivots holds npts entries.  Fails
+ * like the reference ("Cholesky decomposistion failed to converge") when
+ * ceil(npts / block_dim) blocks leave candidates above the threshold, and
+ * when every orbital vanishes at a point. */
+int gi_thc_grid_pivots(gi_context *ctx, int64_t npts, const double *xyz, double qr_thresh,
+                       double qr_thresh_reduced, int64_t block_dim, int64_t *npivots, int64_t *npivots_reduced,
+                       int64_t *pivots);
+
 /* Atomic orbitals on grid points, gridfunc_Orbitals (GridFunctions.f90:154-240)
  * in the Auto2e AO order of the basis, followed when normalize != 0 by
  * thc_normalize_Xgp (TensorHypercontraction.f90:600-637): every row divided

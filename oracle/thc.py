@@ -11,6 +11,9 @@ TEST INFRASTRUCTURE ONLY: imported by tests/, never by the product library.
     z          thc_Z (:393-476); the solve is LAPACK dsysv with uplo "L", the
                routine real_Axb_symmetric_sysv calls (real_linalg.f90:656-703)
     rkab       thc_gammcor_Xga + thc_gammcor_Rkab_2 (THC_Gammcor.f90:82-106, 175-191)
+    grid_pivots  thc_Chol_Pivots + thc_Chol_NextBlock + thc_Chol_Compress
+               (TensorHypercontraction.f90:772-1127): pivoted Cholesky of S built
+               column by column, candidates compressed after every block
 
 Pins (tests/test_oracle_thc.py): the orbitals integrate to the oracle's
 analytic overlap matrix under Gauss-Hermite quadrature (exact for one
@@ -18,7 +21,10 @@ primitive per shell, every l, spherical and Cartesian); a contracted s
 function at its centre equals norm * sum(c); Z equals the least-squares fit
 of R(k, pq) by sum_g X(g,p) X(g,q) Z(g,k) over the FULL square of orbital
 pairs (numpy lstsq, no symmetry folding, no factor 2, no halved diagonal);
-Rkab equals the oracle's MO transform of the fitted vectors.
+Rkab equals the oracle's MO transform of the fitted vectors; grid_pivots
+gives the pivot sequence of a dense textbook pivoted Cholesky of the whole
+S (no blocks, no compression) and leaves every Schur-complement diagonal
+below the threshold.
 """
 from __future__ import annotations
 
@@ -144,3 +150,52 @@ def coefficients_to_auto2e(basis, C_ext, ordering):
     out = np.zeros_like(np.asarray(C_ext, dtype=np.float64))
     out[np.asarray(m) - 1] = np.asarray(C_ext)
     return out * np.asarray(ph)[:, None]
+
+
+def grid_pivots(basis, xyz, qr_thresh, qr_thresh_reduced=-1.0, block_dim=500):
+    """(pivots, npivots_reduced): 0-based indices into xyz of the pruned grid
+    in pivot order, and NGridReduced, as thc_Chol_Pivots computes them.
+    Raises when MaxIters blocks do not converge."""
+    xyz = np.asarray(xyz, dtype=np.float64).reshape(-1, 3)
+    ngrid = xyz.shape[0]
+    thresh = qr_thresh ** 2
+    thresh_red = qr_thresh_reduced ** 2 if qr_thresh_reduced > 0 else -1.0
+    cand = np.arange(ngrid)
+    D = np.ones(ngrid)                 # normalized rows: S(g, g) = 1
+    L = np.zeros((ngrid, 0))           # vectors over the current candidates
+    piv, nred = [], 0
+    max_iters = ngrid // block_dim + (ngrid % block_dim > 0)
+    converged = False
+    for _ in range(max_iters):
+        Xc = orbitals(basis, xyz[cand])
+        max_j = min(block_dim, ngrid - len(piv))
+        cols = []
+        for _k in range(max_j):
+            h = int(np.argmax(D))      # first index of the maximum, as maxloc
+            dh = D[h]
+            if not dh > thresh:
+                break
+            piv.append(int(cand[h]))
+            if dh > thresh_red:
+                nred = len(piv)
+            lj = (Xc @ Xc[h]) ** 2
+            if L.shape[1]:
+                lj -= L @ L[h]
+            if cols:
+                Lc = np.stack(cols, axis=1)
+                lj -= Lc @ Lc[h]
+            lj /= np.sqrt(dh)
+            cols.append(lj)
+            D = D - lj * lj
+        if cols:
+            L = np.concatenate([L, np.stack(cols, axis=1)], axis=1)
+        keep = D > thresh
+        cand, D, L = cand[keep], D[keep], L[keep]
+        if len(cand) == 0:
+            converged = True
+            break
+    if not converged:
+        raise RuntimeError("Cholesky decomposistion failed to converge")
+    if qr_thresh_reduced <= 0:
+        nred = len(piv)
+    return np.array(piv, dtype=np.int64), nred

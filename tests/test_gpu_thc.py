@@ -160,3 +160,34 @@ def test_full_size_normal_equations(ctx):
     Xa = X @ CA
     ref = Z.T @ (Xa[:, 3:7, None] * Xa[:, None, :]).reshape(len(pts), -1)
     assert np.abs(got - ref).max() <= 1e-11 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("qr,block", [(1e-2, 100), (1e-3, 64), (1e-3, 5000)])
+def test_grid_pivots_vs_oracle(ctx, qr, block):
+    """The pruned grid (pivot order and NGridReduced) equals the oracle's
+    restatement of thc_Chol_Pivots on a 1260-point parent grid."""
+    b = systems.water_system(2, 6.0, "small", seed=1)
+    ctx.set_basis(b)
+    pts = systems.atom_centred_points(b, radii=(0.2, 0.5, 0.9, 1.4, 2.0, 3.0, 4.5), ndir=30, seed=5)
+    print("pivots", qr, block, len(pts), flush=True)
+    piv, nred = ctx.thc_grid_pivots(pts, qr, 3 * qr, block)
+    opiv, onred = thc.grid_pivots(b, pts, qr, 3 * qr, block)
+    assert np.array_equal(piv, opiv)
+    assert nred == onred
+    pv2, nred2 = ctx.thc_grid_pivots(pts, qr, -1.0, block)
+    assert np.array_equal(pv2, piv) and nred2 == len(piv)
+
+
+def test_thc_grid_front_end(ctx):
+    b = systems.water_system(1, 6.0, "small", seed=1)
+    ctx.set_basis(b)
+    parent = systems.atom_centred_points(b, radii=(0.3, 0.8, 1.5, 2.5, 4.0), ndir=24, seed=6)
+    X, pts, n, nred = gthc.thc_Grid(ctx, parent, 1e-3, 3e-3, BlockDim=50)
+    assert X.shape == (n, b.nao) and pts.shape == (n, 3) and 0 < nred <= n < len(parent)
+    assert np.allclose(np.linalg.norm(X, axis=1), 1.0, atol=1e-14)
+    ctx.chol_build(tau=1e-6)
+    Z, Zr = ctx.thc_z(X, npts_reduced=nred)
+    Zo, Zro = _oracle_z(ctx, b, X, nred)
+    assert np.abs(Z - Zo).max() <= 1e-6 * np.abs(Zo).max()
+    with pytest.raises(gthc.THCError):
+        gthc.thc_Chol_Pivots(ctx, parent, -1.0)

@@ -180,3 +180,83 @@ def test_rkab_is_mo_transform_of_fitted_vectors(water, ordering):
     finally:
         r.set_rkpq(pyoracle.Oracle(b).chol(tau=1e-6).rkpq)
     assert np.abs(got - ref).max() <= 1e-11 * np.abs(ref).max()
+
+
+def dense_pivots(X, thresh, thresh_red):
+    """Textbook pivoted Cholesky of the whole S = (X X^T)^2: no blocks, no
+    candidate compression; first index of the largest remaining diagonal."""
+    S = (X @ X.T) ** 2
+    n = len(S)
+    D = np.ones(n)
+    cols, piv, nred = [], [], 0
+    while len(piv) < n:
+        h = int(np.argmax(D))
+        if not D[h] > thresh:
+            break
+        c = S[:, h].copy()
+        for v in cols:
+            c -= v * v[h]
+        c /= np.sqrt(D[h])
+        piv.append(h)
+        if D[h] > thresh_red:
+            nred = len(piv)
+        cols.append(c)
+        D -= c * c
+    return np.array(piv), nred
+
+
+@pytest.fixture(scope="module")
+def parent_grid():
+    b = systems.water_system(2, 6.0, "small", seed=1)
+    pts = systems.atom_centred_points(b, radii=(0.2, 0.5, 0.9, 1.4, 2.0, 3.0, 4.5), ndir=30, seed=5)
+    return b, pts
+
+
+@pytest.mark.parametrize("qr,block", [(1e-2, 100), (1e-3, 64), (1e-3, 5000)])
+def test_grid_pivots_equal_dense_pivoted_cholesky(parent_grid, qr, block):
+    b, pts = parent_grid
+    piv, nred = thc.grid_pivots(b, pts, qr, 3 * qr, block_dim=block)
+    X = thc.orbitals(b, pts)
+    dpiv, dnred = dense_pivots(X, qr * qr, 9 * qr * qr)
+    assert np.array_equal(piv, dpiv)
+    assert nred == dnred and 0 < nred < len(piv)
+    # every point left out has its Schur-complement diagonal below the threshold
+    S = (X @ X.T) ** 2
+    rest = np.setdiff1d(np.arange(len(pts)), piv)
+    Spp = S[np.ix_(piv, piv)]
+    Src = S[np.ix_(rest, piv)]
+    schur = 1.0 - np.einsum("ij,ji->i", Src, np.linalg.solve(Spp, Src.T))
+    assert schur.max() <= qr * qr * (1 + 1e-6)
+    # an unsquared metric (the rank-revealing QR of X instead of X(g,pq))
+    # picks another sequence
+    ppiv, _ = dense_pivots_unsquared(X, qr * qr)
+    assert not np.array_equal(ppiv[:len(piv)], piv)
+
+
+def dense_pivots_unsquared(X, thresh):
+    S = X @ X.T
+    D = np.ones(len(S))
+    cols, piv = [], []
+    while len(piv) < len(S):
+        h = int(np.argmax(D))
+        if not D[h] > thresh:
+            break
+        c = S[:, h].copy()
+        for v in cols:
+            c -= v * v[h]
+        c /= np.sqrt(D[h])
+        piv.append(h)
+        cols.append(c)
+        D -= c * c
+    return np.array(piv), len(piv)
+
+
+def test_grid_pivots_no_reduced_and_nonconvergence(parent_grid):
+    b, pts = parent_grid
+    piv, nred = thc.grid_pivots(b, pts, 1e-2, -1.0, block_dim=100)
+    assert nred == len(piv)
+    # MaxIters = ceil(NGrid / BlockDim) blocks of at most BlockDim pivots:
+    # twelve points far apart are all independent and every one is taken
+    few = np.array([[8.0 * i, 0.0, 0.0] for i in range(12)]) + b.atom_coords[0]
+    p2, n2 = thc.grid_pivots(b, few[:1], 1e-2, -1.0, block_dim=5)
+    assert list(p2) == [0] and n2 == 1
