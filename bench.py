@@ -253,15 +253,22 @@ def thc_flops(npts, nvec, nao, npairs):
     return xr, s, solve
 
 
+# radii (bohr) of the synthetic parent grid of the pruning timing: 8 radii x
+# 50 directions per atom
+PARENT_RADII = (0.2, 0.45, 0.8, 1.2, 1.7, 2.4, 3.3, 4.5)
+
+
 def thc_form(hip, reps: int = 3, cpu: bool = True):
     """Tensor hypercontraction on the 1200-AO (H2O)16 system of the headline:
     the decomposition at the THC default threshold (tau = 1e-5), 3072
     atom-centred grid points (48 atoms x 4 radii x 16 directions), then the
     orbitals on the grid, the least-squares Z and Rkab for nocc = 40 x
-    nvirt = 300, all on device pointers; median of reps calls each.  The CPU
-    leg times the oracle's Z (NumPy, threaded BLAS, LAPACK dsysv) on a bounded
-    sample, (H2O)4 with the same basis and grid rules (768 points), beside the
-    device on the same sample."""
+    nvirt = 300, all on device pointers; median of reps calls each; then the
+    pruning of a 19,200-point parent grid (48 atoms x 8 radii x 50
+    directions) at QRThresh 1e-3.  The CPU leg times the oracle's Z (NumPy,
+    threaded BLAS, LAPACK dsysv) and its pruning on a bounded sample, (H2O)4
+    with the same basis and grid rules (768 points; 2880 parent points, 30
+    directions), beside the device on the same sample."""
     from gigpu import lib, systems
 
     def one(basis, npts_dir, reps_):
@@ -320,6 +327,14 @@ def thc_form(hip, reps: int = 3, cpu: bool = True):
         res["rkab_s"] = float(np.median(tr[1:]))
         rk_flops = 2.0 * n * nao * (nocc + nvirt) + 2.0 * nv * n * nocc * nvirt
         res["rkab_tflops"] = rk_flops / res["rkab_s"] / 1e12
+        # grid pruning (thc_Chol_Pivots) of a parent grid at the default
+        # QRThresh = QRThreshReduced = 1e-3, BlockDim = 500
+        parent = systems.atom_centred_points(basis, radii=PARENT_RADII, ndir=50, seed=21)
+        ctx.thc_grid_pivots(parent[:1000], 1e-3, 1e-3, 500)
+        t0 = time.perf_counter()
+        piv, nred = ctx.thc_grid_pivots(parent, 1e-3, 1e-3, 500)
+        res["pruning"] = {"parent_points": len(parent), "ngrid": len(piv), "ngrid_reduced": nred,
+                          "s": time.perf_counter() - t0}
     finally:
         ctx.close()
         for b in bufs:
@@ -345,6 +360,19 @@ def thc_form(hip, reps: int = 3, cpu: bool = True):
         t0 = time.perf_counter()
         thc_oracle.z(X, cols, rk)
         t_cpu = time.perf_counter() - t0
+        sparent = systems.atom_centred_points(small, radii=PARENT_RADII, ndir=30, seed=21)
+        t0 = time.perf_counter()
+        opiv, _ = thc_oracle.grid_pivots(small, sparent, 1e-3, 1e-3, 500)
+        t_cpu_prune = time.perf_counter() - t0
+        pctx = lib.Context(0)
+        try:
+            pctx.set_basis(small)
+            pctx.thc_grid_pivots(sparent[:500], 1e-3, 1e-3, 500)
+            t0 = time.perf_counter()
+            gpiv, _ = pctx.thc_grid_pivots(sparent, 1e-3, 1e-3, 500)
+            t_gpu_prune = time.perf_counter() - t0
+        finally:
+            pctx.close()
         xr, sf, sv = thc_flops(sres["npts"], sres["nvecs"], sres["nao"], sres["norb_pairs"])
         out["cpu_sample"] = {"workload": "the oracle's Z (NumPy + threaded BLAS + LAPACK dsysv) for (H2O)4 (seed 2), "
                                          "same basis, %d points" % sres["npts"],
@@ -352,7 +380,9 @@ def thc_form(hip, reps: int = 3, cpu: bool = True):
                              else int(os.environ["OMP_NUM_THREADS"]),
                              "nao": sres["nao"], "nvecs": sres["nvecs"], "cpu_z_s": t_cpu,
                              "gpu_z_s": sres["z_s"], "cpu_tflops": (xr + sf + sv) / t_cpu / 1e12,
-                             "gpu_over_cpu": t_cpu / sres["z_s"]}
+                             "gpu_over_cpu": t_cpu / sres["z_s"],
+                             "pruning": {"parent_points": len(sparent), "ngrid": len(gpiv), "cpu_s": t_cpu_prune,
+                                         "gpu_s": t_gpu_prune, "same_pivots": bool(np.array_equal(opiv, gpiv))}}
     return out
 
 

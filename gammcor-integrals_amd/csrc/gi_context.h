@@ -168,6 +168,9 @@ struct Context {
   // Coulomb energy / gradient: shell pairs whose density-weighted Schwarz
   // bound on every term is below 10^-grad_screen are left out (0: none)
   long long grad_screen = 0;
+  // THC grid pruning: the explicit Schur-complement form is used once the
+  // candidates' ncand^2 doubles fit this many MiB (0: never)
+  long long thc_schur_mb = 24576;
   // class launches of one ERI request are spread over this many streams (the
   // context stream and eri_streams - 1 side streams forked from it and joined
   // back before the request returns); their destination blocks are disjoint

@@ -44,7 +44,7 @@ struct ThcWork {
   // grid pruning: candidate coordinates and orbitals, residual diagonal,
   // vectors over the candidates (two buffers: compression gathers into the
   // other), staged pivot row, step state, pivots in order
-  DBuf<double> pxyz, pX, pD, pL[2], pxh, plh;
+  DBuf<double> pxyz, pX, pD, pL[2], pS[2], pxh, plh;
   DBuf<long long> pcand, pmap, ppiv;
   DBuf<unsigned char> pstate;
 };
@@ -134,7 +134,16 @@ __global__ void __launch_bounds__(256) k_thc_pair_products(const double *__restr
   }
 }
 
-// ---- grid pruning (pivoted Cholesky of S = (X X^T)^2 with on-the-fly columns)
+// ---- grid pruning (pivoted Cholesky of S = (X X^T)^2, thc_Chol_Pivots)
+// Two ways to form a step's column over the candidates, with the same pivots
+// up to rounding:
+//   on the fly   (X X_h^T)^2 - sum_{j<k} L(:,j) L(h,j) over every earlier vector,
+//                as thc_Chol_NextBlock does: reads X and all of L per pivot;
+//   Schur        Shat(:,h) - sum over the block's vectors only, where Shat is
+//                S less the earlier blocks' vectors, kept explicitly (one DGEMM
+//                per block updates it): reads one column and the block per pivot.
+// The library uses the Schur form when the candidates' Shat fits the
+// thc_schur_mb budget and the on-the-fly form before that.
 struct PivotState {
   long long npiv;   // pivots taken so far (all blocks)
   long long cur;    // vector index of the step's pivot
@@ -147,14 +156,16 @@ struct PivotState {
 
 // One step's pivot: first candidate index of the largest D (maxloc), taken
 // when D > thresh (thc_Chol_NextBlock, TensorHypercontraction.f90:1012-1027);
-// its orbital row and its row of the earlier vectors are staged contiguously.
-// One workgroup of 1024 threads.
+// its orbital row (on-the-fly form, X != null) and its row of the stored
+// vectors L(h, 0 .. cur - lbase - 1) are staged contiguously.  One workgroup
+// of 1024 threads.
 __global__ void __launch_bounds__(1024) k_thc_pivot_select(const double *__restrict__ D, long long ncand,
                                                            const double *__restrict__ X, long long nao,
-                                                           const double *__restrict__ L, double thresh,
-                                                           double thresh_red, const long long *__restrict__ cand,
-                                                           PivotState *st, long long *__restrict__ piv,
-                                                           double *__restrict__ xh, double *__restrict__ lh) {
+                                                           const double *__restrict__ L, long long lbase,
+                                                           double thresh, double thresh_red,
+                                                           const long long *__restrict__ cand, PivotState *st,
+                                                           long long *__restrict__ piv, double *__restrict__ xh,
+                                                           double *__restrict__ lh) {
   __shared__ double s_v[1024];
   __shared__ long long s_i[1024];
   __shared__ int s_take;
@@ -203,42 +214,61 @@ __global__ void __launch_bounds__(1024) k_thc_pivot_select(const double *__restr
   }
   __syncthreads();
   if (!s_take) return;
-  const long long h = s_i[0], k = s_k;
-  for (long long p = t; p < nao; p += 1024) xh[p] = X[h + ncand * p];
-  for (long long j = t; j < k; j += 1024) lh[j] = L[h + ncand * j];
+  const long long h = s_i[0], kl = s_k - lbase;
+  if (X)
+    for (long long p = t; p < nao; p += 1024) xh[p] = X[h + ncand * p];
+  for (long long j = t; j < kl; j += 1024) lh[j] = L[h + ncand * j];
 }
 
-// The step's vector over every candidate g:
-//   L(g, k) = ((sum_p X(g,p) X(h,p))^2 - sum_{j<k} L(g,j) L(h,j)) / sqrt(D(h)),
-//   D(g) -= L(g, k)^2
-// 64 candidates per workgroup, the four wavefronts take every fourth AO and
-// every fourth earlier vector, partial sums added through LDS.
-__global__ void __launch_bounds__(256) k_thc_pivot_column(const double *__restrict__ X, long long ncand,
-                                                          long long nao, double *__restrict__ L,
-                                                          const double *__restrict__ xh,
-                                                          const double *__restrict__ lh, const PivotState *st,
-                                                          double *__restrict__ D) {
-  __shared__ double s1[4][64], s2[4][64];
+// The step's vector over every candidate g (stored at column cur - lbase of L):
+//   on the fly (Sh == null): ((sum_p X(g,p) X(h,p))^2 - sum_j L(g,j) L(h,j)) / sqrt(D(h))
+//   Schur:                    (Sh(g,h) - sum_j L(g,j) L(h,j)) / sqrt(D(h))
+// then D(g) -= L(g, cur)^2.  64 candidates per workgroup; the sixteen
+// wavefronts take every sixteenth AO and stored vector, partial sums added
+// through LDS.
+__global__ void __launch_bounds__(1024) k_thc_pivot_column(const double *__restrict__ X, long long ncand,
+                                                           long long nao, const double *__restrict__ Sh,
+                                                           double *__restrict__ L, long long lbase,
+                                                           const double *__restrict__ xh,
+                                                           const double *__restrict__ lh, const PivotState *st,
+                                                           double *__restrict__ D) {
+  __shared__ double s1[16][64], s2[16][64];
   if (st->done) return;
-  const long long k = st->cur;
+  const long long kl = st->cur - lbase, h = st->h;
   const double dh = st->dh;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const long long g = blockIdx.x * 64LL + lane;
   double a = 0.0, c = 0.0;
   if (g < ncand) {
-    for (long long p = w; p < nao; p += 4) a += X[g + ncand * p] * xh[p];
-    for (long long j = w; j < k; j += 4) c += L[g + ncand * j] * lh[j];
+    if (!Sh)
+      for (long long p = w; p < nao; p += 16) a += X[g + ncand * p] * xh[p];
+    for (long long j = w; j < kl; j += 16) c += L[g + ncand * j] * lh[j];
   }
   s1[w][lane] = a;
   s2[w][lane] = c;
   __syncthreads();
   if (w == 0 && g < ncand) {
-    const double sg = s1[0][lane] + s1[1][lane] + s1[2][lane] + s1[3][lane];
-    const double lg = s2[0][lane] + s2[1][lane] + s2[2][lane] + s2[3][lane];
-    const double v = (sg * sg - lg) / sqrt(dh);
-    L[g + ncand * k] = v;
+    double sg = 0.0, lg = 0.0;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      sg += s1[i][lane];
+      lg += s2[i][lane];
+    }
+    const double col = Sh ? Sh[g + ncand * h] : sg * sg;
+    const double v = (col - lg) / sqrt(dh);
+    L[g + ncand * kl] = v;
     D[g] -= v * v;
   }
+}
+
+// Shat(i, j) of the kept candidates: out(i, j) = in(map[i], map[j])
+__global__ void __launch_bounds__(256) k_thc_gather_square(const long long *__restrict__ map, long long nnew,
+                                                           long long nold, const double *__restrict__ in,
+                                                           double *__restrict__ out) {
+  const long long i = blockIdx.x * 256LL + threadIdx.x;
+  if (i >= nnew) return;
+  const long long si = map[i];
+  for (long long j = blockIdx.y; j < nnew; j += gridDim.y) out[i + nnew * j] = in[si + nold * map[j]];
 }
 
 // compression (thc_Chol_Compress, :1067-1127): candidate rows map[i] of the
@@ -458,6 +488,7 @@ void thc_grid_pivots(Context &C, const double *dxyz, long long npts, double qr_t
   const double thresh = qr_thresh * qr_thresh;
   const double thresh_red = qr_thresh_reduced > 0.0 ? qr_thresh_reduced * qr_thresh_reduced : -1.0;
   const long long max_iters = npts / block_dim + (npts % block_dim > 0);
+  const double schur_bytes = (double)C.thc_schur_mb * 1048576.0;
   long long ncand = npts, npiv = 0;
   w.pcand.alloc((size_t)npts);
   w.pmap.alloc((size_t)npts);
@@ -471,35 +502,63 @@ void thc_grid_pivots(Context &C, const double *dxyz, long long npts, double qr_t
   hipLaunchKernelGGL(k_thc_iota, dim3(g1), dim3(256), 0, C.stream, w.pcand.p, npts);
   hipLaunchKernelGGL(k_thc_fill, dim3(g1), dim3(256), 0, C.stream, w.pD.p, npts, 1.0);
   PivotState st0{};
-  C.h2d(w.pstate.p, &st0, sizeof(st0));
   PivotState *dst = (PivotState *)w.pstate.p;
-  int cur = 0;
-  bool converged = false;
+  int cur = 0, scur = 0;
+  bool converged = false, schur = false;
+  long long lbase = 0;   // first vector index held in L
+  const double one = 1.0, zero = 0.0, mone = -1.0;
   std::vector<double> hD;
   std::vector<long long> map;
   for (long long it = 0; it < max_iters; ++it) {
-    // orbitals of the candidates, normalized (thc_Chol_Pivots :893-895)
-    hipLaunchKernelGGL(k_thc_gather_xyz, dim3((unsigned)((ncand + 255) / 256)), dim3(256), 0, C.stream, dxyz,
-                       w.pcand.p, ncand, w.pxyz.p);
-    w.pX.alloc((size_t)(ncand * nao));
-    thc_orbitals(C, w.pxyz.p, ncand, 1, w.pX.p, ncand);
+    if (!schur) {
+      // orbitals of the candidates, normalized (thc_Chol_Pivots :893-895)
+      hipLaunchKernelGGL(k_thc_gather_xyz, dim3((unsigned)((ncand + 255) / 256)), dim3(256), 0, C.stream, dxyz,
+                         w.pcand.p, ncand, w.pxyz.p);
+      w.pX.alloc((size_t)(ncand * nao));
+      thc_orbitals(C, w.pxyz.p, ncand, 1, w.pX.p, ncand);
+      if ((double)ncand * (double)ncand * 8.0 <= schur_bytes) {
+        // Shat = (X X^T)^2 - L L^T over the candidates; L is dropped
+        w.pS[scur].alloc((size_t)(ncand * ncand));
+        THC_BLAS(rocblas_dgemm(C.blas, rocblas_operation_none, rocblas_operation_transpose, ncand, ncand, nao, &one,
+                               w.pX.p, ncand, w.pX.p, ncand, &zero, w.pS[scur].p, ncand));
+        hipLaunchKernelGGL(k_thc_square, dim3((unsigned)((ncand * ncand + 255) / 256)), dim3(256), 0, C.stream,
+                           w.pS[scur].p, ncand, ncand);
+        GI_HIP(hipGetLastError());
+        if (npiv > 0)
+          THC_BLAS(rocblas_dgemm(C.blas, rocblas_operation_none, rocblas_operation_transpose, ncand, ncand, npiv,
+                                 &mone, w.pL[cur].p, ncand, w.pL[cur].p, ncand, &one, w.pS[scur].p, ncand));
+        schur = true;
+        lbase = npiv;
+      }
+    }
     const long long max_j = std::min(block_dim, npts - npiv);
-    w.pL[cur].grow_keep((size_t)(ncand * (npiv + max_j)), C.stream);
+    if (schur)
+      w.pL[cur].alloc((size_t)(ncand * std::max<long long>(max_j, 1)));
+    else
+      w.pL[cur].grow_keep((size_t)(ncand * (npiv + max_j)), C.stream);
+    if (schur) lbase = npiv;   // the block's vectors only
     // the state's done flag starts clear for every block
     st0.npiv = npiv;
+    st0.done = 0;
     C.h2d(w.pstate.p, &st0, sizeof(st0));
     const unsigned gc = (unsigned)((ncand + 63) / 64);
+    const double *Xs = schur ? nullptr : w.pX.p;
+    const double *Sh = schur ? w.pS[scur].p : nullptr;
     for (long long k = 0; k < max_j; ++k) {
-      hipLaunchKernelGGL(k_thc_pivot_select, dim3(1), dim3(1024), 0, C.stream, w.pD.p, ncand, w.pX.p, nao,
-                         w.pL[cur].p, thresh, thresh_red, w.pcand.p, dst, w.ppiv.p, w.pxh.p, w.plh.p);
-      hipLaunchKernelGGL(k_thc_pivot_column, dim3(gc), dim3(256), 0, C.stream, w.pX.p, ncand, nao, w.pL[cur].p,
-                         w.pxh.p, w.plh.p, dst, w.pD.p);
+      hipLaunchKernelGGL(k_thc_pivot_select, dim3(1), dim3(1024), 0, C.stream, w.pD.p, ncand, Xs, nao, w.pL[cur].p,
+                         lbase, thresh, thresh_red, w.pcand.p, dst, w.ppiv.p, w.pxh.p, w.plh.p);
+      hipLaunchKernelGGL(k_thc_pivot_column, dim3(gc), dim3(1024), 0, C.stream, w.pX.p, ncand, nao, Sh,
+                         w.pL[cur].p, lbase, w.pxh.p, w.plh.p, dst, w.pD.p);
     }
     GI_HIP(hipGetLastError());
     C.read(&st0, w.pstate.p, sizeof(st0));
+    const long long nblk = st0.npiv - npiv;
     npiv = st0.npiv;
     nred = st0.nred;
-    st0.done = 0;
+    // Schur form: Shat -= L_blk L_blk^T
+    if (schur && nblk > 0)
+      THC_BLAS(rocblas_dgemm(C.blas, rocblas_operation_none, rocblas_operation_transpose, ncand, ncand, nblk, &mone,
+                             w.pL[cur].p, ncand, w.pL[cur].p, ncand, &one, w.pS[scur].p, ncand));
     // compression: keep the candidates with D > thresh, in order
     hD.resize((size_t)ncand);
     C.read(hD.data(), w.pD.p, sizeof(double) * ncand);
@@ -513,14 +572,22 @@ void thc_grid_pivots(Context &C, const double *dxyz, long long npts, double qr_t
     const long long nnew = (long long)map.size();
     w.pmap.upload(map, C.stream);
     const int nxt = 1 - cur;
-    w.pL[nxt].alloc((size_t)(nnew * std::max<long long>(npiv, 1)));
+    const long long nkeep = schur ? 0 : npiv;   // vectors carried to the next block
+    w.pL[nxt].alloc((size_t)(nnew * std::max<long long>(nkeep, 1)));
     DBuf<double> Dn;
     DBuf<long long> cn;
     Dn.alloc((size_t)nnew);
     cn.alloc((size_t)nnew);
-    hipLaunchKernelGGL(k_thc_compress, dim3((unsigned)((nnew + 255) / 256), ygrid(npiv)), dim3(256), 0, C.stream,
-                       w.pmap.p, nnew, ncand, npiv, w.pL[cur].p, w.pL[nxt].p, w.pD.p, Dn.p, w.pcand.p, cn.p);
+    hipLaunchKernelGGL(k_thc_compress, dim3((unsigned)((nnew + 255) / 256), ygrid(nkeep)), dim3(256), 0, C.stream,
+                       w.pmap.p, nnew, ncand, nkeep, w.pL[cur].p, w.pL[nxt].p, w.pD.p, Dn.p, w.pcand.p, cn.p);
     GI_HIP(hipGetLastError());
+    if (schur) {
+      w.pS[1 - scur].alloc((size_t)(nnew * nnew));
+      hipLaunchKernelGGL(k_thc_gather_square, dim3((unsigned)((nnew + 255) / 256), ygrid(nnew)), dim3(256), 0,
+                         C.stream, w.pmap.p, nnew, ncand, w.pS[scur].p, w.pS[1 - scur].p);
+      GI_HIP(hipGetLastError());
+      scur = 1 - scur;
+    }
     GI_HIP(hipMemcpyAsync(w.pD.p, Dn.p, sizeof(double) * nnew, hipMemcpyDeviceToDevice, C.stream));
     GI_HIP(hipMemcpyAsync(w.pcand.p, cn.p, sizeof(long long) * nnew, hipMemcpyDeviceToDevice, C.stream));
     cur = nxt;

@@ -108,6 +108,9 @@ int gi_eri_pair_block(gi_context *ctx, int64_t nrow_pairs, const int64_t *row_pa
  *   "vec_blocks"     block rows of the triangular Inv_L Vabrs^T product of
  *                    the full-dimension vectors (20; at most NVecs / 256, each
  *                    a multiple of 128 rows counted from the last row)
+ *   "thc_schur_mb"   gi_thc_grid_pivots keeps the Schur complement of S over
+ *                    the candidates explicitly once it fits this many MiB
+ *                    (24576; 0: columns built on the fly throughout)
  * Unknown names fail.  Results do not depend on these settings beyond
  * floating-point rounding. */
 int gi_set_option(gi_context *ctx, const char *name, int64_t value);

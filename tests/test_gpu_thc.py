@@ -162,19 +162,27 @@ def test_full_size_normal_equations(ctx):
     assert np.abs(got - ref).max() <= 1e-11 * np.abs(ref).max()
 
 
+@pytest.mark.parametrize("schur_mb", [24576, 0, 5])
 @pytest.mark.parametrize("qr,block", [(1e-2, 100), (1e-3, 64), (1e-3, 5000)])
-def test_grid_pivots_vs_oracle(ctx, qr, block):
+def test_grid_pivots_vs_oracle(ctx, qr, block, schur_mb):
     """The pruned grid (pivot order and NGridReduced) equals the oracle's
-    restatement of thc_Chol_Pivots on a 1260-point parent grid."""
+    restatement of thc_Chol_Pivots on a 1260-point parent grid, with the
+    explicit Schur complement from the start (the default budget), with
+    on-the-fly columns throughout (budget 0) and with a switch from the
+    one to the other once fewer than 809 candidates remain (5 MiB)."""
     b = systems.water_system(2, 6.0, "small", seed=1)
     ctx.set_basis(b)
+    ctx.set_option("thc_schur_mb", schur_mb)
     pts = systems.atom_centred_points(b, radii=(0.2, 0.5, 0.9, 1.4, 2.0, 3.0, 4.5), ndir=30, seed=5)
-    print("pivots", qr, block, len(pts), flush=True)
-    piv, nred = ctx.thc_grid_pivots(pts, qr, 3 * qr, block)
+    print("pivots", qr, block, schur_mb, len(pts), flush=True)
+    try:
+        piv, nred = ctx.thc_grid_pivots(pts, qr, 3 * qr, block)
+        pv2, nred2 = ctx.thc_grid_pivots(pts, qr, -1.0, block)
+    finally:
+        ctx.set_option("thc_schur_mb", 24576)
     opiv, onred = thc.grid_pivots(b, pts, qr, 3 * qr, block)
     assert np.array_equal(piv, opiv)
     assert nred == onred
-    pv2, nred2 = ctx.thc_grid_pivots(pts, qr, -1.0, block)
     assert np.array_equal(pv2, piv) and nred2 == len(piv)
 
 
