@@ -356,7 +356,9 @@ __global__ void __launch_bounds__(64) MD4_ATTR md4_kernel(MdArgs A, const ColTil
       }
     } else {
       // contracted pairs: per bra pair and bra primitive pair, X summed over
-      // the ket primitive pairs; the block accumulates over bra primitives
+      // the ket primitive pairs; the block accumulates over bra primitives.
+      // The Boys values of up to M4_NB ket primitive steps are evaluated in
+      // one pass (one lane per step and quartet) before their R tables.
       int maxnp = 1;
       for (int qq = 0; qq < nq; ++qq) maxnp = imax(maxnp, A.crec[pick4(T.jq, qq)].nprim);
       for (int b = 0; b < nb; ++b) {
@@ -368,19 +370,26 @@ __global__ void __launch_bounds__(64) MD4_ATTR md4_kernel(MdArgs A, const ColTil
           load_rows(0);
 #pragma unroll
           for (int hg = 0; hg < NHG; ++hg) X[hg] = 0.0;
-          for (int kp = 0; kp < maxnp; ++kp) {
+          for (int kp0 = 0; kp0 < maxnp; kp0 += M4_NB) {
+            const int nkk = imin(M4_NB, maxnp - kp0);
             __syncthreads();   // earlier readers of the R tables and s_F are done
-            unsigned act = 0;
-            for (int qq = 0; qq < nq; ++qq)
-              if (kp < A.crec[pick4(T.jq, qq)].nprim) act |= 1u << qq;
-            if (lane < nq && ((act >> lane) & 1u)) {
-              const QRec CQ = A.crec[pick4(T.jq, lane)];
-              const PrimRec PK = kp == 0 ? first_prim(CQ) : A.prim[CQ.prim0 + kp];
-              md4_boys<L>(PB, PK, A.kappa, A.boys, s_F[0][lane], s_PQ[0][lane]);
+            if (lane < nkk * nq) {
+              const int kk = lane / nq, q = lane - kk * nq, kp = kp0 + kk;
+              const QRec CQ = A.crec[pick4(T.jq, q)];
+              if (kp < CQ.nprim) {
+                const PrimRec PK = kp == 0 ? first_prim(CQ) : A.prim[CQ.prim0 + kp];
+                md4_boys<L>(PB, PK, A.kappa, A.boys, s_F[kk][q], s_PQ[kk][q]);
+              }
             }
-            build_r(0, nq, act);
-            load_eb(kp);
-            stage1();
+            for (int kk = 0; kk < nkk; ++kk) {
+              const int kp = kp0 + kk;
+              unsigned act = 0;
+              for (int qq = 0; qq < nq; ++qq)
+                if (kp < A.crec[pick4(T.jq, qq)].nprim) act |= 1u << qq;
+              build_r(kk, nq, act);
+              load_eb(kp);
+              stage1();
+            }
           }
           stage2(RB, bp > 0);
         }

@@ -115,3 +115,50 @@ def test_cartesian_mo_transform(gpu_ctx, oracle_mod):
     m = gpu_ctx.chol_meta()
     Rg = full_vectors(m["shell_pair_loc"], m["shell_pair_dim"], gpu_ctx.chol_rkpq(), m["norb_pairs"])
     assert np.all(np.isfinite(Rg))
+
+
+def _deep_basis(spher):
+    """Shells contracted from five to seven primitives (more ket primitive
+    steps than one Boys pass covers) beside single primitives, s through f."""
+    from gigpu.basis import build_basis
+    rng = np.random.default_rng(17)
+
+    def shell(l, n, a0):
+        ex = [a0 * 2.7 ** k for k in range(n)][::-1]
+        return (l, ex, list(rng.uniform(0.2, 0.8, n)))
+    shells = {"O": [shell(0, 7, 0.15), shell(0, 1, 0.3), shell(1, 6, 0.2), shell(1, 1, 0.4), shell(2, 5, 0.3),
+                    shell(3, 1, 0.6)],
+              "H": [shell(0, 5, 0.1), shell(1, 1, 0.5)]}
+    coords = np.array([[0.0, 0.0, 0.0], [1.43, 1.1, 0.0], [-1.43, 1.1, 0.2], [0.3, -0.2, 3.1]])
+    return build_basis(["O", "H", "H", "O"], coords, shells, spher_ao=spher)
+
+
+@pytest.mark.parametrize("name,spher", BASES, ids=[n for n, _ in BASES])
+def test_deeply_contracted_pair_block(gpu_ctx, oracle_mod, name, spher):
+    """Product launches on shells of five to seven primitives: the ket
+    primitive steps run in more than one Boys pass."""
+    b = _deep_basis(spher)
+    o = oracle_mod.Oracle(b)
+    gpu_ctx.set_basis(b)
+    ns = b.nshells
+    allp = [(A, B) for A in range(ns) for B in range(A + 1)]
+    try:
+        gpu_ctx.set_option("eri_min_tiles", 1)
+        V = gpu_ctx.eri_pair_block(allp, allp)
+    finally:
+        gpu_ctx.set_option("eri_min_tiles", 8192)
+    nang = b.shell_nang
+    r0 = 0
+    worst = 0.0
+    for (A, B) in allp:
+        nr = nang(A) * nang(B)
+        c0 = 0
+        for (Cs, D) in allp:
+            nc = nang(Cs) * nang(D)
+            ref = o.eri(A, B, Cs, D).transpose(1, 0, 3, 2).reshape(nr, nc)
+            err = float(np.abs(V[r0:r0 + nr, c0:c0 + nc] - ref).max())
+            worst = max(worst, err)
+            assert err <= TOL, (A, B, Cs, D, err)
+            c0 += nc
+        r0 += nr
+    print("%s: %d x %d pairs, max |gpu - oracle| = %.2e" % (name, len(allp), len(allp), worst))
