@@ -277,11 +277,11 @@ def contracted_form(reps: int = 2):
 def thc_flops(npts, nvec, nao, npairs):
     """Algorithmic FP64 operations of one least-squares Z: XR over the npairs
     stored orbital pairs of the decomposition (the pairs that survive its
-    screening), S = (X X^T)^2, LU of S and the two triangular solves with
-    nvec right-hand sides."""
+    screening), S = (X X^T)^2, the Cholesky factor of S and the two
+    triangular solves with nvec right-hand sides."""
     xr = 2.0 * npts * nvec * npairs
     s = 2.0 * npts * npts * nao
-    solve = 2.0 / 3.0 * npts ** 3 + 2.0 * npts * npts * nvec
+    solve = npts ** 3 / 3.0 + 2.0 * npts * npts * nvec
     return xr, s, solve
 
 

@@ -223,6 +223,7 @@ int gi_set_option(gi_context *cp, const char *name, int64_t value) {
     else if (n == "micro_persist_max") C.micro_persist_max = std::max<int64_t>(0, value);
     else if (n == "grad_screen") C.grad_screen = std::min<int64_t>(30, std::max<int64_t>(0, value));
     else if (n == "thc_schur_mb") C.thc_schur_mb = std::max<int64_t>(0, value);
+    else if (n == "thc_cholesky") C.thc_cholesky = value != 0;
     else if (n == "eri_lane_emax") C.eri_lane_emax = std::max<int64_t>(0, value);
     else if (n == "vec_blocks") C.vec_blocks = std::max<int64_t>(1, value);
     else if (n == "eri_streams")

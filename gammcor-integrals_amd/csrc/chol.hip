@@ -601,6 +601,20 @@ static void chol_inv_rec(Context &C, double *V, double *X, long long n, long lon
                               m2, &zero, X21, n));
 }
 
+// Cholesky factor of V (n x n, lower triangle read, L written over it) and its
+// inverse into X (n x n, zeroed here); W: (n/2 + POTF_NB)^2 doubles, info: one
+// device int.  Returns false when a pivot is not positive (V not numerically
+// positive definite).  For callers outside the decomposition (the THC metric).
+bool chol_factor_inverse(Context &C, double *V, double *X, long long n, double *W, int *info) {
+  hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, C.stream, info, 0);
+  GI_HIP(hipMemsetAsync(X, 0, sizeof(double) * (size_t)n * n, C.stream));
+  chol_inv_rec(C, V, X, n, 0, n, W, info);
+  int h = 0;
+  C.read(&h, info, sizeof(h));
+  return h == 0;
+}
+long long chol_factor_inverse_scratch(long long n) { return (n / 2 + POTF_NB) * (n / 2 + POTF_NB); }
+
 // V (n x n, lower triangle significant) -> L^-1; throws the reference's error
 // when V is not positive definite
 static void factor_and_invert(Context &C, DBuf<double> &V, long long n) {

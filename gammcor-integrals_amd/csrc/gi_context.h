@@ -171,6 +171,9 @@ struct Context {
   // THC grid pruning: the explicit Schur-complement form is used once the
   // candidates' ncand^2 doubles fit this many MiB (0: never)
   long long thc_schur_mb = 24576;
+  // THC least-squares Z: Cholesky of the metric first (LU when a pivot is not
+  // positive); 0 always solves by LU
+  int thc_cholesky = 1;
   // class launches of one ERI request are spread over this many streams (the
   // context stream and eri_streams - 1 side streams forked from it and joined
   // back before the request returns); their destination blocks are disjoint
@@ -239,6 +242,11 @@ void release_chol_work(Context &C);
 
 // max and min over BASE orbital pairs of (pq|pq) - sum_k Rkpq(k, pq)^2 (chol.hip)
 void chol_residual(Context &C, double *rmax, double *rmin);
+// Cholesky factor and inverse of a symmetric positive definite n x n matrix
+// (chol.hip): L over V's lower triangle, L^-1 into X; false when a pivot is
+// not positive.  W holds chol_factor_inverse_scratch(n) doubles.
+bool chol_factor_inverse(Context &C, double *V, double *X, long long n, double *W, int *info);
+long long chol_factor_inverse_scratch(long long n);
 
 // one md_kernel<E, F> launch (eri.hip)
 void launch_md_class(int E, int F, const MdArgs &a, int grid, hipStream_t s);

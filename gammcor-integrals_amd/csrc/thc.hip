@@ -17,10 +17,14 @@
 //   S(g, h)  = (sum_p X(g,p) X(h,p))^2.
 // XR runs over the stored Rkpq columns in chunks: one kernel writes the
 // products of a chunk (zero for padding columns), one dgemm adds
-// 2 Xgpq R^T.  S is a dgemm and an elementwise square.  The symmetric
-// indefinite solve of the reference (LAPACK dsysv) is replaced by LU with
-// partial pivoting (rocsolver_dgesv): both are backward stable, so the two
-// solutions agree to cond(S) times the rounding of the arithmetic.
+// 2 Xgpq R^T.  S is a dgemm and an elementwise square.  The reference solves
+// with the symmetric indefinite LAPACK dsysv.  S is positive semidefinite (the
+// elementwise square of a Gram matrix, Schur product theorem), so the library
+// first factors it by the pivot-block Cholesky of the decomposition (factor
+// and inverse, then Z = L^-T (L^-1 XR), two dgemms; the reduced grid's factor
+// is the leading block of the same one), and solves by LU with partial
+// pivoting (rocsolver_dgesv) when a pivot is not positive.  The solutions
+// agree with the reference's to cond(S) times the rounding of the arithmetic.
 #include <algorithm>
 #include <cmath>
 #include <rocsolver/rocsolver.h>
@@ -40,7 +44,7 @@ namespace gi {
 struct ThcWork {
   DBuf<int> shl, aoloc, flag, ipiv, info;
   DBuf<int> colpq;                    // (p, q) of every stored Rkpq column, -1 for padding
-  DBuf<double> W, S, Sr, CA, CB, Xa, Xb, part;
+  DBuf<double> W, S, Sr, CA, CB, Xa, Xb, part, Li, Lw;
   // grid pruning: candidate coordinates and orbitals, residual diagonal,
   // vectors over the candidates (two buffers: compression gathers into the
   // other), staged pivot row, step state, pivots in order
@@ -421,6 +425,30 @@ void thc_z(Context &C, const double *dX, long long ldx, long long npts, long lon
     w.Sr.alloc((size_t)(nred * nred));
     GI_HIP(hipMemcpy2DAsync(w.Sr.p, sizeof(double) * nred, w.S.p, sizeof(double) * npts, sizeof(double) * nred, nred,
                             hipMemcpyDeviceToDevice, C.stream));
+  }
+  if (nvec > 0 && C.thc_cholesky) {
+    // Cholesky: L over S, L^-1 into Li; Z = Li^T (Li XR) with XR from dZ
+    w.Li.alloc((size_t)(npts * npts));
+    w.Lw.alloc((size_t)std::max(chol_factor_inverse_scratch(npts), npts * nvec));
+    w.info.alloc(1);
+    if (chol_factor_inverse(C, w.S.p, w.Li.p, npts, w.Lw.p, w.info.p)) {
+      auto apply = [&](long long n, double *Z, long long ld) {
+        THC_BLAS(rocblas_dgemm(C.blas, rocblas_operation_none, rocblas_operation_none, n, nvec, n, &one, w.Li.p,
+                               npts, Z, ld, &zero, w.Lw.p, n));
+        THC_BLAS(rocblas_dgemm(C.blas, rocblas_operation_transpose, rocblas_operation_none, n, nvec, n, &one,
+                               w.Li.p, npts, w.Lw.p, n, &zero, Z, ld));
+      };
+      apply(npts, dZ, ldz);
+      if (reduced) apply(nred, dZr, ldzr);
+      C.wait();
+      return;
+    }
+    // not positive definite: rebuild S (the factorization overwrote it) for LU
+    THC_BLAS(rocblas_dgemm(C.blas, rocblas_operation_none, rocblas_operation_transpose, npts, npts, nao, &one, dX,
+                           ldx, dX, ldx, &zero, w.S.p, npts));
+    hipLaunchKernelGGL(k_thc_square, dim3((unsigned)((npts * npts + 255) / 256)), dim3(256), 0, C.stream, w.S.p,
+                       npts, npts);
+    GI_HIP(hipGetLastError());
   }
   if (nvec > 0) {
     thc_solve(C, w.S.p, npts, dZ, ldz, nvec);

@@ -111,6 +111,8 @@ int gi_eri_pair_block(gi_context *ctx, int64_t nrow_pairs, const int64_t *row_pa
  *   "thc_schur_mb"   gi_thc_grid_pivots keeps the Schur complement of S over
  *                    the candidates explicitly once it fits this many MiB
  *                    (24576; 0: columns built on the fly throughout)
+ *   "thc_cholesky"   gi_thc_z factors S by Cholesky first, LU when a pivot
+ *                    is not positive (1); 0: LU always
  * Unknown names fail.  Results do not depend on these settings beyond
  * floating-point rounding. */
 int gi_set_option(gi_context *ctx, const char *name, int64_t value);
@@ -299,7 +301,9 @@ int gi_thc_orbitals_device(gi_context *ctx, int64_t npts, const double *xyz, int
  * (thc_S, :580-597).  When 0 < npts_reduced < npts, Zgk_reduced
  * (npts_reduced, NVecs) solves the same system on the first npts_reduced
  * points (:455-466); otherwise it is not written.  Needs gi_chol_build
- * first.  Fails when S is exactly singular.  Host pointers. */
+ * first.  S is factored by Cholesky, or by LU with partial pivoting where
+ * a Cholesky pivot is not positive (the reference's dsysv is the symmetric
+ * indefinite solver); fails when S is exactly singular.  Host pointers. */
 int gi_thc_z(gi_context *ctx, int64_t npts, const double *Xgp, int64_t ldx, int64_t npts_reduced, double *Zgk,
              int64_t ldz, double *Zgk_reduced, int64_t ldzr);
 /* Same with device pointers for Xgp, Zgk and Zgk_reduced. */

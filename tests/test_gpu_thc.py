@@ -50,23 +50,29 @@ def _oracle_z(ctx, b, X, nred=0):
     return thc.z(X, cols, ctx.chol_rkpq(), nred)
 
 
+@pytest.mark.parametrize("solver", ["cholesky", "lu"])
 @pytest.mark.parametrize("ndir,tol", [(6, 1e-9), (12, 1e-5)])
-def test_z_vs_oracle(ctx, ndir, tol):
+def test_z_vs_oracle(ctx, ndir, tol, solver):
     """Z from the device's own decomposition against the oracle's dsysv
-    solve of the same system: LU and Bunch-Kaufman agree to about
-    cond(S) * 1e-16 (cond(S) ~ 1e5 for 72 points, ~ 1e9 for 144)."""
+    solve of the same system, by the library's Cholesky of S and by its LU
+    (option thc_cholesky = 0): the solvers agree to about cond(S) * 1e-16
+    (cond(S) ~ 1e5 for 72 points, ~ 1e9 for 144)."""
     b = systems.water_system(1, 6.0, "small", seed=1)
     ctx.set_basis(b)
     ctx.chol_build(tau=1e-6)
     pts = systems.atom_centred_points(b, ndir=ndir, seed=3)
-    print("z", ndir, len(pts), flush=True)
+    print("z", ndir, len(pts), solver, flush=True)
     X = ctx.thc_orbitals(pts)
-    Z, Zr = ctx.thc_z(X, npts_reduced=len(pts) // 2)
+    ctx.set_option("thc_cholesky", 1 if solver == "cholesky" else 0)
+    try:
+        Z, Zr = ctx.thc_z(X, npts_reduced=len(pts) // 2)
+        # npts_reduced >= npts: no reduced system, the same Z
+        Z2, Zr2 = ctx.thc_z(X, npts_reduced=len(pts))
+    finally:
+        ctx.set_option("thc_cholesky", 1)
     Zo, Zro = _oracle_z(ctx, b, X, len(pts) // 2)
     assert np.abs(Z - Zo).max() <= tol * np.abs(Zo).max()
     assert np.abs(Zr - Zro).max() <= tol * np.abs(Zro).max()
-    # npts_reduced >= npts: no reduced system
-    Z2, Zr2 = ctx.thc_z(X, npts_reduced=len(pts))
     assert Zr2 is None and np.array_equal(Z2, Z)
 
 
