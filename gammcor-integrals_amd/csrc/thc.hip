@@ -45,10 +45,11 @@ struct ThcWork {
   DBuf<int> shl, aoloc, flag, ipiv, info;
   DBuf<int> colpq;                    // (p, q) of every stored Rkpq column, -1 for padding
   DBuf<double> W, S, Sr, CA, CB, Xa, Xb, part, Li, Lw;
-  // grid pruning: candidate coordinates and orbitals, residual diagonal,
-  // vectors over the candidates (two buffers: compression gathers into the
-  // other), staged pivot row, step state, pivots in order
-  DBuf<double> pxyz, pX, pD, pL[2], pS[2], pxh, plh;
+  // grid pruning: candidate coordinates and orbitals, residual diagonal (two
+  // buffers, read by a step from one and written to the other), vectors and
+  // Schur complements over the candidates (two buffers each: compression
+  // gathers into the other), step state, pivots in order
+  DBuf<double> pxyz, pX, pD, pD2, pL[2], pS[2];
   DBuf<long long> pcand, pmap, ppiv;
   DBuf<unsigned char> pstate;
 };
@@ -158,28 +159,33 @@ struct PivotState {
   int pad;
 };
 
-// One step's pivot: first candidate index of the largest D (maxloc), taken
-// when D > thresh (thc_Chol_NextBlock, TensorHypercontraction.f90:1012-1027);
-// its orbital row (on-the-fly form, X != null) and its row of the stored
-// vectors L(h, 0 .. cur - lbase - 1) are staged contiguously.  One workgroup
-// of 1024 threads.
-__global__ void __launch_bounds__(1024) k_thc_pivot_select(const double *__restrict__ D, long long ncand,
-                                                           const double *__restrict__ X, long long nao,
-                                                           const double *__restrict__ L, long long lbase,
-                                                           double thresh, double thresh_red,
-                                                           const long long *__restrict__ cand, PivotState *st,
-                                                           long long *__restrict__ piv, double *__restrict__ xh,
-                                                           double *__restrict__ lh) {
+// One pivot step, launched with vector index k.  Every workgroup finds the
+// step's pivot itself, the first candidate index of the largest D (maxloc), so
+// that no workgroup waits for another; it is taken when D > thresh
+// (thc_Chol_NextBlock, TensorHypercontraction.f90:1012-1027), and workgroup 0
+// records it.  Then the vector over every candidate g (column k - lbase of L)
+//   on the fly (Sh == null): ((sum_p X(g,p) X(h,p))^2 - sum_j L(g,j) L(h,j)) / sqrt(D(h))
+//   Schur:                    (Sh(g,h) - sum_j L(g,j) L(h,j)) / sqrt(D(h))
+// and Dout(g) = Din(g) - L(g,k)^2: D is read from one buffer and written to the
+// other, so that the workgroups still searching see the step's input.  64
+// candidates per workgroup; the sixteen wavefronts split the sums.  A step
+// without a pivot sets done, and every later step of the block exits at once.
+__global__ void __launch_bounds__(1024) k_thc_pivot_step(const double *__restrict__ X, long long ncand,
+                                                         long long nao, const double *__restrict__ Sh,
+                                                         double *__restrict__ L, long long lbase, long long k,
+                                                         const double *__restrict__ Din, double *__restrict__ Dout,
+                                                         const long long *__restrict__ cand,
+                                                         long long *__restrict__ piv, PivotState *st, double thresh,
+                                                         double thresh_red) {
   __shared__ double s_v[1024];
   __shared__ long long s_i[1024];
-  __shared__ int s_take;
-  __shared__ long long s_k;
+  __shared__ double s1[16][64], s2[16][64];
   const int t = threadIdx.x;
-  if (st->done) return;   // uniform: written only by thread 0 of an earlier launch
+  if (st->done) return;   // uniform: set by an earlier launch
   double bv = -1.0;
   long long bi = -1;
   for (long long g = t; g < ncand; g += 1024) {
-    const double v = D[g];
+    const double v = Din[g];
     if (bi < 0 || v > bv) {   // strictly larger: the first index wins within a thread
       bv = v;
       bi = g;
@@ -199,54 +205,25 @@ __global__ void __launch_bounds__(1024) k_thc_pivot_select(const double *__restr
     }
     __syncthreads();
   }
-  if (t == 0) {
-    const double dh = s_v[0];
-    const int take = s_i[0] >= 0 && dh > thresh;
-    s_take = take;
-    if (take) {
-      const long long k = st->npiv;
-      s_k = k;
-      st->cur = k;
-      st->npiv = k + 1;
-      st->h = s_i[0];
-      st->dh = dh;
-      piv[k] = cand[s_i[0]];
-      if (dh > thresh_red) st->nred = k + 1;
-    } else {
-      st->done = 1;
-    }
+  const double dh = s_v[0];
+  const long long h = s_i[0];
+  if (!(h >= 0 && dh > thresh)) {   // uniform: the same LDS values in every lane
+    if (blockIdx.x == 0 && t == 0) st->done = 1;
+    return;
   }
-  __syncthreads();
-  if (!s_take) return;
-  const long long h = s_i[0], kl = s_k - lbase;
-  if (X)
-    for (long long p = t; p < nao; p += 1024) xh[p] = X[h + ncand * p];
-  for (long long j = t; j < kl; j += 1024) lh[j] = L[h + ncand * j];
-}
-
-// The step's vector over every candidate g (stored at column cur - lbase of L):
-//   on the fly (Sh == null): ((sum_p X(g,p) X(h,p))^2 - sum_j L(g,j) L(h,j)) / sqrt(D(h))
-//   Schur:                    (Sh(g,h) - sum_j L(g,j) L(h,j)) / sqrt(D(h))
-// then D(g) -= L(g, cur)^2.  64 candidates per workgroup; the sixteen
-// wavefronts take every sixteenth AO and stored vector, partial sums added
-// through LDS.
-__global__ void __launch_bounds__(1024) k_thc_pivot_column(const double *__restrict__ X, long long ncand,
-                                                           long long nao, const double *__restrict__ Sh,
-                                                           double *__restrict__ L, long long lbase,
-                                                           const double *__restrict__ xh,
-                                                           const double *__restrict__ lh, const PivotState *st,
-                                                           double *__restrict__ D) {
-  __shared__ double s1[16][64], s2[16][64];
-  if (st->done) return;
-  const long long kl = st->cur - lbase, h = st->h;
-  const double dh = st->dh;
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (blockIdx.x == 0 && t == 0) {
+    piv[k] = cand[h];
+    st->npiv = k + 1;
+    if (dh > thresh_red) st->nred = k + 1;
+  }
+  const long long kl = k - lbase;
+  const int lane = t & 63, w = t >> 6;
   const long long g = blockIdx.x * 64LL + lane;
   double a = 0.0, c = 0.0;
   if (g < ncand) {
     if (!Sh)
-      for (long long p = w; p < nao; p += 16) a += X[g + ncand * p] * xh[p];
-    for (long long j = w; j < kl; j += 16) c += L[g + ncand * j] * lh[j];
+      for (long long p = w; p < nao; p += 16) a += X[g + ncand * p] * X[h + ncand * p];
+    for (long long j = w; j < kl; j += 16) c += L[g + ncand * j] * L[h + ncand * j];
   }
   s1[w][lane] = a;
   s2[w][lane] = c;
@@ -261,7 +238,7 @@ __global__ void __launch_bounds__(1024) k_thc_pivot_column(const double *__restr
     const double col = Sh ? Sh[g + ncand * h] : sg * sg;
     const double v = (col - lg) / sqrt(dh);
     L[g + ncand * kl] = v;
-    D[g] -= v * v;
+    Dout[g] = Din[g] - v * v;
   }
 }
 
@@ -502,9 +479,8 @@ void thc_rkab(Context &C, const double *dX, long long ldx, long long npts, const
 }
 
 // thc_Chol_Pivots (TensorHypercontraction.f90:772-968) on the device: blocks of
-// at most block_dim pivots, each step one select and one column launch (a
-// step after the last pivot exits at once), one synchronization per block for
-// the compression.  dxyz: the parent points (3, npts), device.
+// at most block_dim pivots, one launch per step (a step after the last pivot
+// exits at once), one synchronization per block for the compression.  dxyz: the parent points (3, npts), device.
 void thc_grid_pivots(Context &C, const double *dxyz, long long npts, double qr_thresh, double qr_thresh_reduced,
                      long long block_dim, std::vector<long long> &pivots, long long &nred) {
   if (!C.basis_set) throw Error{4, "basis not set"};
@@ -521,9 +497,10 @@ void thc_grid_pivots(Context &C, const double *dxyz, long long npts, double qr_t
   w.pcand.alloc((size_t)npts);
   w.pmap.alloc((size_t)npts);
   w.pD.alloc((size_t)npts);
+  w.pD2.alloc((size_t)npts);
+  double *const Dbuf[2] = {w.pD.p, w.pD2.p};
+  int dcur = 0;   // the buffer holding the current residual diagonal
   w.ppiv.alloc((size_t)npts);
-  w.pxh.alloc((size_t)nao);
-  w.plh.alloc((size_t)std::max<long long>(npts, 1));
   w.pstate.alloc(sizeof(PivotState));
   w.pxyz.alloc((size_t)(3 * npts));
   const unsigned g1 = (unsigned)((npts + 255) / 256);
@@ -570,17 +547,15 @@ void thc_grid_pivots(Context &C, const double *dxyz, long long npts, double qr_t
     st0.done = 0;
     C.h2d(w.pstate.p, &st0, sizeof(st0));
     const unsigned gc = (unsigned)((ncand + 63) / 64);
-    const double *Xs = schur ? nullptr : w.pX.p;
     const double *Sh = schur ? w.pS[scur].p : nullptr;
-    for (long long k = 0; k < max_j; ++k) {
-      hipLaunchKernelGGL(k_thc_pivot_select, dim3(1), dim3(1024), 0, C.stream, w.pD.p, ncand, Xs, nao, w.pL[cur].p,
-                         lbase, thresh, thresh_red, w.pcand.p, dst, w.ppiv.p, w.pxh.p, w.plh.p);
-      hipLaunchKernelGGL(k_thc_pivot_column, dim3(gc), dim3(1024), 0, C.stream, w.pX.p, ncand, nao, Sh,
-                         w.pL[cur].p, lbase, w.pxh.p, w.plh.p, dst, w.pD.p);
-    }
+    for (long long k = 0; k < max_j; ++k)
+      hipLaunchKernelGGL(k_thc_pivot_step, dim3(gc), dim3(1024), 0, C.stream, w.pX.p, ncand, nao, Sh, w.pL[cur].p,
+                         lbase, npiv + k, Dbuf[(dcur + k) & 1], Dbuf[(dcur + k + 1) & 1], w.pcand.p, w.ppiv.p, dst,
+                         thresh, thresh_red);
     GI_HIP(hipGetLastError());
     C.read(&st0, w.pstate.p, sizeof(st0));
     const long long nblk = st0.npiv - npiv;
+    dcur = (int)((dcur + nblk) & 1);   // the buffer the block's last vector wrote
     npiv = st0.npiv;
     nred = st0.nred;
     // Schur form: Shat -= L_blk L_blk^T
@@ -589,7 +564,7 @@ void thc_grid_pivots(Context &C, const double *dxyz, long long npts, double qr_t
                              w.pL[cur].p, ncand, w.pL[cur].p, ncand, &one, w.pS[scur].p, ncand));
     // compression: keep the candidates with D > thresh, in order
     hD.resize((size_t)ncand);
-    C.read(hD.data(), w.pD.p, sizeof(double) * ncand);
+    C.read(hD.data(), Dbuf[dcur], sizeof(double) * ncand);
     map.clear();
     for (long long g = 0; g < ncand; ++g)
       if (hD[g] > thresh) map.push_back(g);
@@ -602,12 +577,12 @@ void thc_grid_pivots(Context &C, const double *dxyz, long long npts, double qr_t
     const int nxt = 1 - cur;
     const long long nkeep = schur ? 0 : npiv;   // vectors carried to the next block
     w.pL[nxt].alloc((size_t)(nnew * std::max<long long>(nkeep, 1)));
-    DBuf<double> Dn;
     DBuf<long long> cn;
-    Dn.alloc((size_t)nnew);
     cn.alloc((size_t)nnew);
     hipLaunchKernelGGL(k_thc_compress, dim3((unsigned)((nnew + 255) / 256), ygrid(nkeep)), dim3(256), 0, C.stream,
-                       w.pmap.p, nnew, ncand, nkeep, w.pL[cur].p, w.pL[nxt].p, w.pD.p, Dn.p, w.pcand.p, cn.p);
+                       w.pmap.p, nnew, ncand, nkeep, w.pL[cur].p, w.pL[nxt].p, Dbuf[dcur], Dbuf[1 - dcur], w.pcand.p,
+                       cn.p);
+    dcur = 1 - dcur;
     GI_HIP(hipGetLastError());
     if (schur) {
       w.pS[1 - scur].alloc((size_t)(nnew * nnew));
@@ -616,7 +591,6 @@ void thc_grid_pivots(Context &C, const double *dxyz, long long npts, double qr_t
       GI_HIP(hipGetLastError());
       scur = 1 - scur;
     }
-    GI_HIP(hipMemcpyAsync(w.pD.p, Dn.p, sizeof(double) * nnew, hipMemcpyDeviceToDevice, C.stream));
     GI_HIP(hipMemcpyAsync(w.pcand.p, cn.p, sizeof(long long) * nnew, hipMemcpyDeviceToDevice, C.stream));
     cur = nxt;
     ncand = nnew;
