@@ -461,34 +461,77 @@ __global__ void __launch_bounds__(256) k_chol_inv_leaf(double *__restrict__ V, d
   if (tid == 0) s_info = *info;
   __syncthreads();
   if (s_info != 0) return;
-  for (int e = tid; e < POTF_NB * POTF_NB; e += 256) {
-    const int r = e % POTF_NB, c = e / POTF_NB;
-    s[r][c] = (r < b && c < b && r >= c) ? V[(k0 + r) + n * (k0 + c)] : (r == c ? 1.0 : 0.0);
-    x[r][c] = 0.0;
-  }
-  __syncthreads();
+  // factorization in registers: thread (tr, tc) of a 16 x 16 grid holds the
+  // 4 x 4 tile of rows 4 tr .. 4 tr + 3 and columns 4 tc .. 4 tc + 3 (tiles
+  // above the diagonal idle); per column j the diagonal goes through LDS, the
+  // owners of column j scale it and publish it, every tile takes its rank-1
+  // update: two barriers per column.  The block is padded with an identity.
+  const int tr = tid & 15, tc = tid >> 4;
+  double a[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int jj = 0; jj < 4; ++jj) {
+      const int r = 4 * tr + i, c = 4 * tc + jj;
+      a[i][jj] = (r < b && c < b && r >= c) ? V[(k0 + r) + n * (k0 + c)] : (r == c ? 1.0 : 0.0);
+    }
+  for (int e = tid; e < POTF_NB * POTF_NB; e += 256) x[e % POTF_NB][e / POTF_NB] = 0.0;
+  __shared__ double s_d;
+  __shared__ double s_col[POTF_NB];
   int jend = b;
   for (int j = 0; j < b; ++j) {
-    const double d = s[j][j];
+    const int tj = j >> 2;
+    if (tr == tj && tc == tj) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+        if ((j & 3) == i) s_d = a[i][i];
+    }
+    __syncthreads();
+    const double d = s_d;
     if (!(d > 0.0)) {   // the same LDS value in every lane: a uniform exit
       jend = j;
       break;
     }
     const double sq = sqrt(d);
-    __syncthreads();   // every lane has read s[j][j]
-    for (int r = j + tid; r < b; r += 256) s[r][j] = r == j ? sq : s[r][j] / sq;
-    __syncthreads();
-    const int m = b - j - 1;
-    for (int e = tid; e < m * m; e += 256) {
-      const int rr = j + 1 + e % m, cc = j + 1 + e / m;
-      if (rr >= cc) s[rr][cc] -= s[rr][j] * s[cc][j];
+    if (tc == tj && tr >= tj) {
+#pragma unroll
+      for (int jj = 0; jj < 4; ++jj)
+        if ((j & 3) == jj) {
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const int r = 4 * tr + i;
+            if (r == j) a[i][jj] = sq;
+            else if (r > j) a[i][jj] = a[i][jj] / sq;
+            if (r >= j) s_col[r] = a[i][jj];
+          }
+        }
     }
     __syncthreads();
+    if (tr >= tc && 4 * tr + 3 > j) {
+      double cr[4], cc[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        cr[i] = s_col[4 * tr + i];
+        cc[i] = s_col[4 * tc + i];
+      }
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int jj = 0; jj < 4; ++jj) {
+          const int r = 4 * tr + i, c = 4 * tc + jj;
+          if (c > j && r >= c) a[i][jj] -= cr[i] * cc[jj];
+        }
+    }
   }
   if (jend < b) {
     if (tid == 0) *info = (int)(k0 + jend + 1);
     return;
   }
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int jj = 0; jj < 4; ++jj) s[4 * tr + i][4 * tc + jj] = a[i][jj];
+  __syncthreads();
   // X = L^-1 (L padded to POTF_NB rows with an identity): the eight 8 x 8
   // diagonal blocks by forward substitution, one column per lane, then pairs
   // of blocks merged, [A 0; B C]^-1 = [A^-1 0; -C^-1 (B A^-1) C^-1], for
