@@ -5,6 +5,7 @@ THC driver of TensorHypercontraction.f90 over the device library.
     thc_Grid              TensorHypercontraction.f90:300-390 from the caller's parent grid
     thc_CoulombMatrix     thc_CoulombMatrix_QuadraticMemory (TensorHypercontraction.f90:232-272)
                           with the grid points given by the caller
+    thc_ReduceGrid        TensorHypercontraction.f90:275-297
     thc_gammcor_XZ        THC_Gammcor.f90:154-253 (accuracy presets, Omega -> Kappa)
     thc_gammcor_Rkab      THC_Gammcor.f90:25-80
 
@@ -105,6 +106,17 @@ def thc_CoulombMatrix(AOBasis, GridXYZ, CholeskyTauThresh: float, Kappa: float =
     finally:
         if own:
             ctx.close()
+
+
+def thc_ReduceGrid(THCGrid: TCoulTHCGrid) -> None:
+    """Keep only the first NGridReduced points: Zgk <- ZgkReduced, Xgp
+    truncated, Zgh recomputed (in place)."""
+    if THCGrid.NGridReduced < THCGrid.NGrid:
+        n = THCGrid.NGridReduced
+        THCGrid.NGrid = n
+        THCGrid.Zgk, THCGrid.ZgkReduced = THCGrid.ZgkReduced, None
+        THCGrid.Xgp = THCGrid.Xgp[:n].copy(order="F")
+        THCGrid.Zgh = THCGrid.Zgk @ THCGrid.Zgk.T
 
 
 def thc_gammcor_XZ(AOBasis, ParentXYZ, Accuracy: int, CholeskyThresh: float | None = None,

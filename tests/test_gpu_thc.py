@@ -111,6 +111,10 @@ def test_python_front_end(ctx):
     g = gthc.thc_CoulombMatrix(b, pts, 1e-6, NGridReduced=30, ctx=ctx)
     assert g.NGrid == len(pts) and g.NGridReduced == 30 and g.ZgkReduced.shape[0] == 30
     assert np.allclose(g.Zgh, g.Zgk @ g.Zgk.T, rtol=0, atol=1e-12 * np.abs(g.Zgh).max())
+    Zr = g.ZgkReduced.copy()
+    gthc.thc_ReduceGrid(g)
+    assert g.NGrid == 30 and np.array_equal(g.Zgk, Zr) and g.Xgp.shape == (30, b.nao)
+    assert np.allclose(g.Zgh, Zr @ Zr.T, rtol=0, atol=1e-12 * np.abs(g.Zgh).max())
     CA = np.random.default_rng(2).standard_normal((b.nao, 6))
     R = gthc.thc_gammcor_Rkab(CA, 1, 6, CA, 1, 6, b, pts, gthc.THC_ACCURACY_TIGHT, lib.ORDERING_ORCA, ctx=ctx)
     assert R.shape[1] == 36
