@@ -209,3 +209,29 @@ def test_thc_grid_front_end(ctx):
     assert np.abs(Z - Zo).max() <= 1e-6 * np.abs(Zo).max()
     with pytest.raises(gthc.THCError):
         gthc.thc_Chol_Pivots(ctx, parent, -1.0)
+
+
+def test_cartesian_z_rkab_and_pivots(ctx):
+    """Cartesian contracted AOs (Molpro ordering for Rkab, the only external
+    order the reference defines for Cartesian functions): pruning, Z and
+    Rkab against the oracle."""
+    b = systems.water_system(1, 6.0, "small", seed=7, spher_ao=False, contracted=True)
+    ctx.set_basis(b)
+    parent = systems.atom_centred_points(b, radii=(0.3, 0.8, 1.5, 2.5, 4.0), ndir=20, seed=4)
+    piv, nred = ctx.thc_grid_pivots(parent, 1e-2, -1.0, 40)
+    opiv, _ = thc.grid_pivots(b, parent, 1e-2, -1.0, 40)
+    assert np.array_equal(piv, opiv) and nred == len(piv)
+    pts = parent[piv][:60]
+    # Cartesian d shells make the reference's Step-2 decode meet a singular
+    # pivot block; the Step-1 decode factorizes the Step-1 pivots
+    ctx.chol_build(tau=1e-6, flags=lib.CHOL_DECODE_FIXED)
+    X = ctx.thc_orbitals(pts)
+    assert np.abs(X - thc.orbitals(b, pts)).max() <= 1e-13
+    Z, _ = ctx.thc_z(X)
+    Zo, _ = _oracle_z(ctx, b, X)
+    assert np.abs(Z - Zo).max() <= 1e-7 * np.abs(Zo).max()
+    C = np.random.default_rng(5).standard_normal((b.nao, 5))
+    got = ctx.thc_rkab(X, Z, C, 1, 5, C, 2, 4, aomap.MOLPRO)
+    Co = thc.coefficients_to_auto2e(b, C, aomap.MOLPRO)
+    ref = thc.rkab(X, Z, Co, 1, 5, Co, 2, 4)
+    assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max()
