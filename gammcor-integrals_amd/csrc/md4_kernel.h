@@ -121,9 +121,28 @@ __device__ __forceinline__ void md4_boys(const PrimRec &PB, const PrimRec &PK, d
 
 // PLAIN: every pair of the launch has one primitive pair (uncontracted), so the
 // kernel omits the primitive loops
+// Waves per SIMD asked of the compiler for the classes whose register count
+// lay a few registers above the next occupancy step: it then keeps the MFMA
+// accumulators in VGPRs instead of AGPRs and reaches the step with no
+// scratch.  Measured per class on the (H2O)16 builds:
+//   uncontracted, three waves: (1,6) -21 %, (2,5) -12 %, (2,6) -7 %, (5,2) -6 %,
+//     (4,4) -2 %; four waves for (2,3), (3,2), (3,4), (4,0), (4,1) or three for
+//     (0,6), (3,5), (5,0) changed nothing or slowed them ((3,4) +15 %);
+//   contracted, four waves for (0,3), (1,3), (2,0..2), three for (0,4), (2,3),
+//     (2,4), (3,2), (3,4), (4,2), (4,3): 8 to 17 % each, none slower.
+// Higher orders spill under either request.
+__host__ __device__ constexpr int md4_min_waves(int E, int F, bool plain) {
+  if (plain) return ((E == 1 && F == 6) || (E == 2 && F >= 5) || (E == 4 && F == 4) || (E == 5 && F == 2)) ? 3 : 1;
+  return ((E == 0 && F == 3) || (E == 1 && F == 3) || (E == 2 && F <= 2)) ? 4
+         : ((E == 0 && F == 4) || (E == 2 && (F == 3 || F == 4)) || (E == 3 && (F == 2 || F == 4)) ||
+            (E == 4 && (F == 2 || F == 3)))
+             ? 3
+             : 1;
+}
+
 template <int E, int F, bool PLAIN>
-__global__ void __launch_bounds__(64) MD4_ATTR md4_kernel(MdArgs A, const ColTile *__restrict__ ctiles, long long nct,
-                                                           const QRec *__restrict__ rrec) {
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(md4_min_waves(E, F, PLAIN)))) MD4_ATTR
+md4_kernel(MdArgs A, const ColTile *__restrict__ ctiles, long long nct, const QRec *__restrict__ rrec) {
   using S = Md4<E, F>;
   constexpr int L = S::L, L1 = S::L1, NHE = S::NHE, NHF = S::NHF, NHL = S::NHL, NHG = S::NHG, KSF = S::KSF;
   constexpr int DS = S::DS, QM = S::QM, NL1 = S::NL1;
