@@ -251,12 +251,14 @@ def dense_pivots_unsquared(X, thresh):
     return np.array(piv), len(piv)
 
 
-def test_grid_pivots_no_reduced_and_nonconvergence(parent_grid):
+def test_grid_pivots_without_reduced_threshold_and_last_block(parent_grid):
     b, pts = parent_grid
     piv, nred = thc.grid_pivots(b, pts, 1e-2, -1.0, block_dim=100)
     assert nred == len(piv)
-    # MaxIters = ceil(NGrid / BlockDim) blocks of at most BlockDim pivots:
-    # twelve points far apart are all independent and every one is taken
-    few = np.array([[8.0 * i, 0.0, 0.0] for i in range(12)]) + b.atom_coords[0]
-    p2, n2 = thc.grid_pivots(b, few[:1], 1e-2, -1.0, block_dim=5)
-    assert list(p2) == [0] and n2 == 1
+    # six independent points, blocks of four: MaxIters = ceil(6 / 4) = 2 and
+    # the second block may take only NGrid - NPivots = 2 more pivots; every
+    # point is taken, in the order of the dense pivoted Cholesky
+    six = b.atom_coords + np.array([0.3, -0.2, 0.25])
+    p6, n6 = thc.grid_pivots(b, six, 1e-2, -1.0, block_dim=4)
+    d6, _ = dense_pivots(thc.orbitals(b, six), 1e-4, -1.0)
+    assert sorted(p6) == list(range(6)) and n6 == 6 and np.array_equal(p6, d6)
