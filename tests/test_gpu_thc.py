@@ -235,3 +235,39 @@ def test_cartesian_z_rkab_and_pivots(ctx):
     Co = thc.coefficients_to_auto2e(b, C, aomap.MOLPRO)
     ref = thc.rkab(X, Z, Co, 1, 5, Co, 2, 4)
     assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max()
+
+
+def test_thc_error_paths(ctx):
+    """Argument errors come back as statuses with the reference's messages
+    where it has one, and nothing is written."""
+    import ctypes as C
+    b = systems.water_system(1, 6.0, "small", seed=1)
+    ctx.set_basis(b)
+    ctx.chol_build(tau=1e-6)
+    pts = systems.atom_centred_points(b, ndir=4, seed=1)
+    X = np.asfortranarray(ctx.thc_orbitals(pts))
+    n, nv = X.shape[0], ctx.chol_dims()["nvecs"]
+    L = ctx.L
+    Z = np.zeros((n, nv), order="F")
+    dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))
+    # leading dimensions smaller than the number of points
+    assert L.gi_thc_z(ctx.h, n, dp(X), n - 1, 0, dp(Z), n, None, 1) == 81
+    assert L.gi_thc_z(ctx.h, n, dp(X), n, 0, dp(Z), n - 1, None, 1) == 81
+    assert L.gi_thc_z(ctx.h, 0, dp(X), n, 0, dp(Z), n, None, 1) == 81
+    assert not Z.any()
+    # Rkab: empty column ranges, short CA
+    CA = np.zeros((b.nao, 4), order="F")
+    out = np.zeros(nv * 16)
+    Zr, _ = ctx.thc_z(X)
+    Zr = np.asfortranarray(Zr)
+    assert L.gi_thc_rkab(ctx.h, n, dp(X), n, dp(Zr), n, nv, dp(CA), b.nao, 3, 2, dp(CA), b.nao, 1, 4, 0,
+                         dp(out)) == 51
+    assert L.gi_thc_rkab(ctx.h, n, dp(X), n, dp(Zr), n, nv, dp(CA), b.nao - 1, 1, 4, dp(CA), b.nao, 1, 4, 0,
+                         dp(out)) == 52
+    # pruning: thresholds and block size
+    with pytest.raises(lib.GigpuError, match="QR threshold"):
+        ctx.thc_grid_pivots(pts, 0.0)
+    with pytest.raises(lib.GigpuError, match="block dimension"):
+        ctx.thc_grid_pivots(pts, 1e-3, -1.0, 0)
+    with pytest.raises(lib.GigpuError, match="Invalid external orbital ordering|ordering"):
+        ctx.thc_rkab(X, Zr, CA, 1, 4, CA, 1, 4, 9)
