@@ -65,7 +65,27 @@ typedef struct gi_context gi_context;
 int gi_create(gi_context **ctx, int64_t device);
 int gi_destroy(gi_context *ctx);
 /* Message of the last failure on this thread (replaces msg(...,MSG_ERROR) +
- * error stop of the reference). */
+ * error stop of the reference).  Status codes, by subsystem:
+ *    1  invalid device index             2  HIP / rocBLAS / rocSOLVER call failed
+ *    3  shell pair beyond the ERI kernels 4  basis not set
+ *    5  block sizes not positive         6  tau not positive
+ *    7  null context                     8  invalid argument (shells, pairs, sizes)
+ *    9  l beyond 5, or unknown option   10  nonpositive maximum diagonal integral
+ *   20  vector limit hit before tau     21  batch too large for the micro iterations
+ *   30  invalid pivot array             40  Cholesky factorization of V(P,P) failed
+ *   50  no decomposition (build first)  51  empty or invalid Rkab range
+ *   52  CA/CB leading dimension < nao   53  buffer too small for Rpq -> Rab
+ *   60  no ERI kernel for the orders    61  null density or energy pointer
+ *   62  gradient beyond its l limit     63  invalid split-product arguments
+ *   64  one-electron integrals, l > 3   65  leading dimension < nao
+ *   70  unknown AO ordering             71  AO map not a permutation
+ *   72  in-place AO reordering          73  AO reordering beyond the scratch limit
+ *   74  scratch array not set up        80  grid orbitals with l > 5
+ *   81  invalid grid size / leading dim 82  a grid point where every orbital vanishes
+ *   83  THC metric S singular           84  invalid QR threshold or block size
+ *   85  grid pruning did not converge   99  C++ exception (message attached)
+ * The message text names the failing check; where the reference stops with a
+ * message of its own, the text is that message. */
 const char *gi_last_error(void);
 
 /* Upload the basis: the TAOBasis components (basis_definitions.f90:94-192)
